@@ -1,0 +1,1114 @@
+/*
+ * alma_oracle.cpp -- FP64 CPU restatement of the reference protein_physics
+ * energy and MC path.  TEST INFRASTRUCTURE: used only by tests/, by the smoke
+ * entry point and by the benchmark's CPU-baseline leg, as the checker.
+ *
+ * Parity pinning (see tests/test_oracle_*.py):
+ *   - golden vectors produced by the reference itself compiled from its own
+ *     sources (oracle/Makefile -> oracle/_ref), committed under tests/golden;
+ *   - closed forms (isolated atom Born / SASA, two-particle pair energy,
+ *     dihedral relation), invariants (rigid-motion invariance, NL equals brute
+ *     force, incremental dE equals full difference), the C++ standard's
+ *     mt19937 known answer, and finite differences for the gradient.
+ */
+#include "alma_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// Constants: src/physics_engine.cpp:181-196, 1959-1961
+constexpr double COULOMB = 332.0636;
+constexpr double EPS_WATER = 78.5;
+constexpr double EPS_PROT = 1.0;
+constexpr double KAPPA = 0.1257;
+constexpr double GAMMA_SA = 0.00542;
+constexpr double BETA_SA = 0.92;
+constexpr double PROBE_R = 1.4;
+constexpr double NL_RCUT2 = (12.0 + 2.0) * (12.0 + 2.0);
+constexpr double PAIR_CUT2 = 12.0 * 12.0;
+constexpr double HALF_SKIN2 = 1.0;
+constexpr double HARD_SCALE = 1.0e4;
+constexpr double HARD_CAP = 5.0e3;
+constexpr double HARD_FRAC = 0.6;
+constexpr double GB_COEF = -0.5 * (1.0 / EPS_PROT - 1.0 / EPS_WATER) * COULOMB;
+constexpr double K_SS = 600.0;
+constexpr double R0_SS = 2.044;
+constexpr double PI = 3.14159265358979323846;
+
+struct Sys {
+    int n;
+    const double* q;
+    const double* rs;
+    const double* ep;
+};
+
+// ───────────────────────── RNG (mt19937 + libstdc++ 11) ─────────────────────
+// std::mt19937 per [rand.eng.mers]; uniform_int_distribution downscaling by
+// Lemire's method as in /usr/include/c++/11/bits/uniform_int_dist.h:240-268;
+// generate_canonical<double,53> from two words (bits/random.tcc:3348-3378).
+struct MT {
+    uint32_t s[624];
+    int idx;
+    int64_t used = 0;
+    explicit MT(uint32_t seed) {
+        s[0] = seed;
+        for (int i = 1; i < 624; ++i)
+            s[i] = 1812433253u * (s[i - 1] ^ (s[i - 1] >> 30)) + (uint32_t)i;
+        idx = 624;
+    }
+    void twist() {
+        for (int i = 0; i < 624; ++i) {
+            uint32_t y = (s[i] & 0x80000000u) | (s[(i + 1) % 624] & 0x7fffffffu);
+            uint32_t v = s[(i + 397) % 624] ^ (y >> 1);
+            if (y & 1u) v ^= 0x9908b0dfu;
+            s[i] = v;
+        }
+        idx = 0;
+    }
+    uint32_t operator()() {
+        if (idx >= 624) twist();
+        uint32_t y = s[idx++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        ++used;
+        return y;
+    }
+    // uniform_int_distribution<int>(0, hi)
+    int uniform_int(int hi) {
+        uint32_t range = (uint32_t)hi + 1u;
+        uint64_t prod = (uint64_t)(*this)() * (uint64_t)range;
+        uint32_t low = (uint32_t)prod;
+        if (low < range) {
+            uint32_t thr = (uint32_t)(-range) % range;
+            while (low < thr) {
+                prod = (uint64_t)(*this)() * (uint64_t)range;
+                low = (uint32_t)prod;
+            }
+        }
+        return (int)(prod >> 32);
+    }
+    double canonical() {
+        double sum = (double)(*this)();
+        sum += (double)(*this)() * 4294967296.0;
+        double r = sum / 18446744073709551616.0;
+        if (r >= 1.0) r = std::nextafter(1.0, 0.0);
+        return r;
+    }
+    // uniform_real_distribution<double>(a, b)
+    double uniform_real(double a, double b) { return canonical() * (b - a) + a; }
+};
+
+// ───────────────────────────── neighbour list ───────────────────────────────
+// NeighborList::build, src/physics_engine.cpp:293-342: cells of side
+// sqrt(196)+1e-6, box grown by one cell, 27-cell scan dz->dy->dx, j>i,
+// d^2 < 196 strict, per-atom list in scan order.
+struct NL {
+    std::vector<int> off, js;
+    std::vector<double> ref;
+    void build(int n, const double* x) {
+        off.assign(n + 1, 0);
+        js.clear();
+        ref.assign(x, x + 3 * (size_t)n);
+        if (n <= 1) return;
+        double lo[3] = {x[0], x[1], x[2]}, hi[3] = {x[0], x[1], x[2]};
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], x[3 * i + d]);
+                hi[d] = std::max(hi[d], x[3 * i + d]);
+            }
+        const double cs = std::sqrt(NL_RCUT2) + 1e-6;
+        int nc[3];
+        for (int d = 0; d < 3; ++d) {
+            double len = hi[d] - (lo[d] - cs) + 2.0 * cs;
+            lo[d] -= cs;
+            nc[d] = std::max(1, (int)std::ceil(len / cs));
+        }
+        std::vector<std::vector<int>> cells((size_t)nc[0] * nc[1] * nc[2]);
+        std::vector<int> cc(3 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            for (int d = 0; d < 3; ++d)
+                cc[3 * i + d] = std::min(nc[d] - 1,
+                                         std::max(0, (int)std::floor((x[3 * i + d] - lo[d]) / cs)));
+            cells[(size_t)cc[3 * i] + nc[0] * ((size_t)cc[3 * i + 1] + nc[1] * (size_t)cc[3 * i + 2])]
+                .push_back(i);
+        }
+        for (int i = 0; i < n; ++i) {
+            for (int dz = -1; dz <= 1; ++dz) {
+                int z = cc[3 * i + 2] + dz;
+                if (z < 0 || z >= nc[2]) continue;
+                for (int dy = -1; dy <= 1; ++dy) {
+                    int y = cc[3 * i + 1] + dy;
+                    if (y < 0 || y >= nc[1]) continue;
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        int xx = cc[3 * i] + dx;
+                        if (xx < 0 || xx >= nc[0]) continue;
+                        for (int j : cells[(size_t)xx + nc[0] * ((size_t)y + nc[1] * (size_t)z)]) {
+                            if (j <= i) continue;
+                            double a = x[3 * i] - x[3 * j], b = x[3 * i + 1] - x[3 * j + 1],
+                                   c = x[3 * i + 2] - x[3 * j + 2];
+                            if (a * a + b * b + c * c < NL_RCUT2) js.push_back(j);
+                        }
+                    }
+                }
+            }
+            off[i + 1] = (int)js.size();
+        }
+    }
+    // NeighborList::needs_rebuild, src/physics_engine.cpp:345-351
+    bool stale(int n, const double* x) const {
+        for (int i = 0; i < n; ++i) {
+            double a = x[3 * i] - ref[3 * i], b = x[3 * i + 1] - ref[3 * i + 1],
+                   c = x[3 * i + 2] - ref[3 * i + 2];
+            if (a * a + b * b + c * c > HALF_SKIN2) return true;
+        }
+        return false;
+    }
+};
+
+inline double dist2(const double* x, int i, int j) {
+    double a = x[3 * i] - x[3 * j], b = x[3 * i + 1] - x[3 * j + 1], c = x[3 * i + 2] - x[3 * j + 2];
+    return a * a + b * b + c * c;
+}
+
+// hct, src/physics_engine.cpp:2007-2011 (non-textbook HCT form, as written)
+inline double hct(double r, double r2, double ri, double rj) {
+    double L = std::max(std::abs(r - rj), ri), U = r + rj;
+    if (ri >= U) return 0.0;
+    return 1.0 / L - 1.0 / U + (r2 - rj * rj + ri * ri) / (2.0 * r * ri * ri) * std::log(L / U) * 0.5 / r;
+}
+
+// born_radii, src/physics_engine.cpp:2024-2039 (both directions over the NL,
+// exclusions and the 12-14 A shell included; a = 1/max(1/R - S/2, 2))
+void born_full(int n, const double* x, const double* rs, const NL& nl, double* a, double* half) {
+    std::vector<double> sum(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double hs = 0.0;
+        for (int k = nl.off[i]; k < nl.off[i + 1]; ++k) {
+            int j = nl.js[k];
+            double r2 = dist2(x, i, j), r = std::sqrt(r2);
+            double hij = hct(r, r2, rs[i], rs[j]);
+            sum[i] += hij;
+            hs += hij;
+            sum[j] += hct(r, r2, rs[j], rs[i]);
+        }
+        if (half) half[i] = hs;
+    }
+    for (int i = 0; i < n; ++i) a[i] = 1.0 / std::max(1.0 / rs[i] - 0.5 * sum[i], 2.0);
+}
+
+// update_born, src/physics_engine.cpp:2043-2051 (half list j > idx only)
+void born_update(int idx, const double* x, const double* rs, const NL& nl, double* a) {
+    double sum = 0.0;
+    for (int k = nl.off[idx]; k < nl.off[idx + 1]; ++k) {
+        int j = nl.js[k];
+        double r2 = dist2(x, idx, j), r = std::sqrt(r2);
+        sum += hct(r, r2, rs[idx], rs[j]);
+    }
+    a[idx] = 1.0 / std::max(1.0 / rs[idx] - 0.5 * sum, 2.0);
+}
+
+// sasa_nonpolar, src/physics_engine.cpp:2068-2083 (half list, NL order fold)
+double sasa_atom(int i, const double* x, const double* rs, const NL& nl) {
+    double ri = rs[i] + PROBE_R, sa = 4.0 * PI * ri * ri;
+    for (int k = nl.off[i]; k < nl.off[i + 1]; ++k) {
+        int j = nl.js[k];
+        double rj = rs[j] + PROBE_R;
+        double r = std::sqrt(dist2(x, i, j)), dc = ri + rj;
+        if (r >= dc) continue;
+        double h = (dc - r) / (2.0 * ri);
+        sa -= std::min(sa * 0.85, 2.0 * PI * ri * ri * h);
+    }
+    return sa;
+}
+double sasa_total(int n, const double* x, const double* rs, const NL& nl) {
+    double E = BETA_SA;
+    for (int i = 0; i < n; ++i) E += GAMMA_SA * std::max(0.0, sasa_atom(i, x, rs, nl));
+    return E;
+}
+
+// pair_e / pair_e_diag, src/physics_engine.cpp:2148-2199
+inline double pair_terms(const double* pi, const double* pj, double qi, double qj, double ri,
+                         double rj, double ei, double ej, double ai, double aj, double* t4) {
+    double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+    double r2 = dx * dx + dy * dy + dz * dz, r = std::sqrt(r2), sig = ri + rj;
+    double qp = qi * qj;
+    double edh = (COULOMB * qp) / (EPS_WATER * r) * std::exp(-KAPPA * r);
+    double fgb = std::sqrt(r2 + ai * aj * std::exp(-r2 / (4.0 * ai * aj)));
+    double egb = GB_COEF * qp / fgb;
+    double eps = std::sqrt(ei * ej), s6 = std::pow(sig / r, 6);
+    double elj = 4.0 * eps * (s6 * s6 - s6);
+    double hc = 0.0;
+    double rc = sig * HARD_FRAC;
+    if (r < rc) {
+        double xx = std::pow(rc / r, 12.0) - 1.0;
+        hc = std::min(HARD_SCALE * xx * xx, HARD_CAP);
+    }
+    if (t4) {
+        t4[0] = edh;
+        t4[1] = egb;
+        t4[2] = elj;
+        t4[3] = hc;
+    }
+    return edh + egb + elj + hc;
+}
+
+// is_excluded, src/physics_engine.cpp:1700-1705
+inline bool excluded(const orc_topo* t, int i, int j) {
+    if (!t || !t->excl_off) return false;
+    if (i > j) std::swap(i, j);
+    const int* b = t->excl_idx + t->excl_off[i];
+    const int* e = t->excl_idx + t->excl_off[i + 1];
+    return std::binary_search(b, e, j);
+}
+
+// dihedral_angle, src/physics_engine.cpp:1871-1885
+inline double dih_angle(const double* x, int a, int b, int c, int d) {
+    double b1[3], b2[3], b3[3];
+    for (int k = 0; k < 3; ++k) {
+        b1[k] = x[3 * a + k] - x[3 * b + k];
+        b2[k] = x[3 * c + k] - x[3 * b + k];
+        b3[k] = x[3 * d + k] - x[3 * c + k];
+    }
+    double n1x = b1[1] * b2[2] - b1[2] * b2[1], n1y = b1[2] * b2[0] - b1[0] * b2[2],
+           n1z = b1[0] * b2[1] - b1[1] * b2[0];
+    double n2x = b2[1] * b3[2] - b2[2] * b3[1], n2y = b2[2] * b3[0] - b2[0] * b3[2],
+           n2z = b2[0] * b3[1] - b2[1] * b3[0];
+    double m1x = n1y * b2[2] - n1z * b2[1], m1y = n1z * b2[0] - n1x * b2[2],
+           m1z = n1x * b2[1] - n1y * b2[0];
+    double xx = n1x * n2x + n1y * n2y + n1z * n2z;
+    double yy = m1x * n2x + m1y * n2y + m1z * n2z;
+    return std::atan2(yy, xx);
+}
+
+// dihedral_e / dihedral_e_boundary, src/physics_engine.cpp:1893-1921
+double dih_energy(const double* x, const orc_topo* t, const std::vector<char>* side) {
+    double E = 0.0;
+    for (int k = 0; k < t->n_dih; ++k) {
+        int a = t->dih_a[k], b = t->dih_b[k], c = t->dih_c[k], d = t->dih_d[k];
+        if (side) {
+            const auto& s = *side;
+            bool any_side = s[a] || s[b] || s[c] || s[d];
+            bool any_fixed = !s[a] || !s[b] || !s[c] || !s[d];
+            if (!any_side || !any_fixed) continue;
+        }
+        double phi = dih_angle(x, a, b, c, d);
+        for (int m = t->dih_toff[k]; m < t->dih_toff[k + 1]; ++m)
+            E += t->dih_v2[m] * (1.0 + std::cos((double)t->dih_n[m] * phi - t->dih_gamma[m]));
+    }
+    return E;
+}
+
+// ss_e / ss_e_side, src/physics_engine.cpp:1965-1994
+double ss_energy(const double* x, const orc_topo* t, const std::vector<char>* side) {
+    double E = 0.0;
+    for (int k = 0; k < t->n_ss; ++k) {
+        int i = t->ss_i[k], j = t->ss_j[k];
+        if (side && !(*side)[i] && !(*side)[j]) continue;
+        double dr = std::sqrt(dist2(x, i, j)) - R0_SS;
+        E += K_SS * dr * dr;
+    }
+    return E;
+}
+
+// total_e, src/physics_engine.cpp:2205-2234, with the per-term split of
+// pair_e_diag.  Term sums are plain FP64 (the reference's OpenMP reduction
+// order is nondeterministic; its total agrees to ~1e-13 relative).
+void total_terms(const Sys& s, const double* x, const NL& nl, const double* a, const orc_topo* t,
+                 double* terms) {
+    for (int k = 0; k < ORC_NTERMS; ++k) terms[k] = 0.0;
+    terms[ORC_SASA] = sasa_total(s.n, x, s.rs, nl);
+    if (t && t->n_dih) terms[ORC_DIH] = dih_energy(x, t, nullptr);
+    if (t && t->n_ss) terms[ORC_SS] = ss_energy(x, t, nullptr);
+    double dh = 0, gb = 0, lj = 0, hc = 0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : dh, gb, lj, hc)
+    for (int i = 0; i < s.n; ++i) {
+        for (int k = nl.off[i]; k < nl.off[i + 1]; ++k) {
+            int j = nl.js[k];
+            if (excluded(t, i, j)) continue;
+            if (dist2(x, i, j) > PAIR_CUT2) continue;
+            double t4[4];
+            pair_terms(x + 3 * i, x + 3 * j, s.q[i], s.q[j], s.rs[i], s.rs[j], s.ep[i], s.ep[j], a[i],
+                       a[j], t4);
+            dh += t4[0];
+            gb += t4[1];
+            lj += t4[2];
+            hc += t4[3];
+        }
+    }
+    terms[ORC_DH] = dh;
+    terms[ORC_GB] = gb;
+    terms[ORC_LJ] = lj;
+    terms[ORC_HC] = hc;
+    terms[ORC_TOTAL] = terms[ORC_SASA] + terms[ORC_DIH] + terms[ORC_SS] + dh + gb + lj + hc;
+}
+
+double total_energy(const Sys& s, const double* x, const NL& nl, const double* a, const orc_topo* t) {
+    double terms[ORC_NTERMS];
+    total_terms(s, x, nl, a, t, terms);
+    return terms[ORC_TOTAL];
+}
+
+// rodrigues, src/physics_engine.cpp:1848-1862
+inline void rodrigues(double* p, const double* o, const double* u, double c, double sn) {
+    double vx = p[0] - o[0], vy = p[1] - o[1], vz = p[2] - o[2];
+    double dot = u[0] * vx + u[1] * vy + u[2] * vz;
+    double cx = u[1] * vz - u[2] * vy, cy = u[2] * vx - u[0] * vz, cz = u[0] * vy - u[1] * vx;
+    double k = 1.0 - c;
+    p[0] = o[0] + vx * c + cx * sn + u[0] * dot * k;
+    p[1] = o[1] + vy * c + cy * sn + u[1] * dot * k;
+    p[2] = o[2] + vz * c + cz * sn + u[2] * dot * k;
+}
+
+// ────────────────────────────────── MC chain ────────────────────────────────
+// generate_ensemble / run_landscape_* chains, src/physics_engine.cpp:2273-2543
+// and :2556-2923.  One object per chain; RNG draws happen in exactly the
+// reference order (draw counts are state dependent).
+struct Chain {
+    Sys s;
+    const orc_topo* t;
+    std::vector<double> x, a;
+    NL nl;
+    MT rng;
+    double curE = 0, cur_max;
+    int acc_win = 0, tot_win = 0;
+    std::vector<char> in_side;
+    std::vector<int> group;
+    int64_t rebuilds = 0, accepted = 0, moves = 0;
+    // last-move record for tracing
+    int last_kind = -1, last_bond = -1;
+    double last_delta = 0, last_dE = 0, last_l1 = 0;
+    int last_acc = 0;
+
+    Chain(const Sys& sys, const double* x0, const orc_topo* topo, uint32_t seed, double max_angle)
+        : s(sys), t(topo), x(x0, x0 + 3 * (size_t)sys.n), a(sys.n), rng(seed), cur_max(max_angle),
+          in_side(sys.n, 0), group(sys.n, 0) {
+        nl.build(s.n, x.data());
+        born_full(s.n, x.data(), s.rs, nl, a.data(), nullptr);
+        curE = total_energy(s, x.data(), nl, a.data(), t);
+    }
+    int side_begin(int k) const { return t->side_off[k]; }
+    int side_end(int k) const { return t->side_off[k + 1]; }
+
+    // Per-item values of one evaluation, kept only when an error scale
+    // sum |new - old| over pairs / atoms / records is wanted.
+    struct Items {
+        std::vector<double> pair, sa, dih, ss;
+    };
+    bool want_items = false;
+
+    // cross_e (src/physics_engine.cpp:2312-2323) generalised to side groups:
+    // pairs with different group labels (cross_e_concerted :2766-2777).
+    double cross(Items* it) const {
+        double E = 0.0;
+        if (it) it->pair.assign(nl.js.size(), 0.0);
+        for (int i = 0; i < s.n; ++i)
+            for (int k = nl.off[i]; k < nl.off[i + 1]; ++k) {
+                int j = nl.js[k];
+                if (group[i] == group[j]) continue;
+                if (excluded(t, i, j)) continue;
+                if (dist2(x.data(), i, j) > PAIR_CUT2) continue;
+                double e = pair_terms(&x[3 * i], &x[3 * j], s.q[i], s.q[j], s.rs[i], s.rs[j], s.ep[i],
+                                      s.ep[j], a[i], a[j], nullptr);
+                E += e;
+                if (it) it->pair[k] = e;
+            }
+        return E;
+    }
+    double sasa(Items* it) const {
+        if (!it) return sasa_total(s.n, x.data(), s.rs, nl);
+        it->sa.assign(s.n, 0.0);
+        double E = BETA_SA;
+        for (int i = 0; i < s.n; ++i) {
+            double v = GAMMA_SA * std::max(0.0, sasa_atom(i, x.data(), s.rs, nl));
+            it->sa[i] = v;
+            E += v;
+        }
+        return E;
+    }
+
+    struct Terms {
+        double cross, sasa, dih, ss;
+    };
+    Terms terms(Items* it) const {
+        Terms r;
+        r.cross = cross(it);
+        r.sasa = sasa(it);
+        r.dih = dih_energy(x.data(), t, &in_side);
+        r.ss = t->n_ss ? ss_energy(x.data(), t, &in_side) : 0.0;
+        if (it) {
+            it->dih.assign(t->n_dih, 0.0);
+            for (int k = 0; k < t->n_dih; ++k) {
+                int q4[4] = {t->dih_a[k], t->dih_b[k], t->dih_c[k], t->dih_d[k]};
+                bool any_side = false, any_fixed = false;
+                for (int m = 0; m < 4; ++m) (in_side[q4[m]] ? any_side : any_fixed) = true;
+                if (!any_side || !any_fixed) continue;
+                double phi = dih_angle(x.data(), q4[0], q4[1], q4[2], q4[3]);
+                for (int m = t->dih_toff[k]; m < t->dih_toff[k + 1]; ++m)
+                    it->dih[k] += t->dih_v2[m] * (1.0 + std::cos((double)t->dih_n[m] * phi - t->dih_gamma[m]));
+            }
+            it->ss.assign(t->n_ss, 0.0);
+            for (int k = 0; k < t->n_ss; ++k) {
+                int i = t->ss_i[k], j = t->ss_j[k];
+                if (!in_side[i] && !in_side[j]) continue;
+                double dr = std::sqrt(dist2(x.data(), i, j)) - R0_SS;
+                it->ss[k] = K_SS * dr * dr;
+            }
+        }
+        return r;
+    }
+    static double l1_of(const Items& o, const Items& nw) {
+        double L = 0.0;
+        auto acc = [&](const std::vector<double>& u, const std::vector<double>& v) {
+            for (size_t k = 0; k < u.size(); ++k) L += std::fabs(v[k] - u[k]);
+        };
+        acc(o.pair, nw.pair);
+        acc(o.sa, nw.sa);
+        acc(o.dih, nw.dih);
+        acc(o.ss, nw.ss);
+        return L;
+    }
+
+    bool metropolis(double dE, double T) { return dE < 0.0 || rng.canonical() < std::exp(-dE / T); }
+
+    void restore(const std::vector<int>& idx, const std::vector<double>& op,
+                 const std::vector<double>& ob) {
+        for (size_t k = 0; k < idx.size(); ++k) {
+            int i = idx[k];
+            x[3 * i] = op[3 * k];
+            x[3 * i + 1] = op[3 * k + 1];
+            x[3 * i + 2] = op[3 * k + 2];
+            a[i] = ob[k];
+        }
+    }
+
+    // shared tail: old terms are taken by the caller, rotation applied by
+    // `move`, then Born update, new terms and Metropolis.
+    template <class Move>
+    std::pair<bool, bool> evaluate(const std::vector<int>& moved, const std::vector<int>& born_set,
+                                   double T, Move&& move) {
+        std::vector<double> op(3 * moved.size()), ob(moved.size());
+        for (size_t k = 0; k < moved.size(); ++k) {
+            int i = moved[k];
+            op[3 * k] = x[3 * i];
+            op[3 * k + 1] = x[3 * i + 1];
+            op[3 * k + 2] = x[3 * i + 2];
+            ob[k] = a[i];
+        }
+        Items io, in;
+        Terms o = terms(want_items ? &io : nullptr);
+        if (!move()) {  // degenerate second axis (crankshaft): positions restored
+            restore(moved, op, ob);
+            return {false, false};
+        }
+        for (int i : born_set) born_update(i, x.data(), s.rs, nl, a.data());
+        Terms nw = terms(want_items ? &in : nullptr);
+        double dE = (nw.cross - o.cross) + (nw.sasa - o.sasa) + (nw.dih - o.dih) + (nw.ss - o.ss);
+        last_dE = dE;
+        last_l1 = want_items ? l1_of(io, in) : 0.0;
+        bool acc = metropolis(dE, T);
+        if (acc)
+            curE += dE;
+        else
+            restore(moved, op, ob);
+        last_acc = acc;
+        return {acc, true};
+    }
+
+    bool axis(int bond, double* o, double* u) const {
+        int bi = t->rb_i[bond], bj = t->rb_j[bond];
+        for (int d = 0; d < 3; ++d) {
+            o[d] = x[3 * bi + d];
+            u[d] = x[3 * bj + d] - x[3 * bi + d];
+        }
+        double l = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        if (l < 1e-10) return false;
+        u[0] /= l;
+        u[1] /= l;
+        u[2] /= l;
+        return true;
+    }
+    std::vector<int> side_of(int k) const {
+        return std::vector<int>(t->side_idx + side_begin(k), t->side_idx + side_end(k));
+    }
+    void mark(const std::vector<int>& v, int g) {
+        for (int i : v) {
+            in_side[i] = g != 0;
+            group[i] = g;
+        }
+    }
+
+    // try_torsion, src/physics_engine.cpp:2328-2399 (also the +-30 deg mode)
+    std::pair<bool, bool> torsion(double T, bool fixed30) {
+        last_kind = 0;
+        int k = rng.uniform_int(t->n_rot - 1);
+        last_bond = k;
+        if (side_begin(k) == side_end(k)) return {false, false};
+        double o[3], u[3];
+        if (!axis(k, o, u)) return {false, false};
+        double delta;
+        if (fixed30) {
+            delta = rng.uniform_real(-PI / 6.0, PI / 6.0);
+        } else {
+            int kind = t->rb_kind[k];
+            double base = (kind == 0 || kind == 1) ? cur_max : cur_max * 2.5;
+            base *= t->rb_scale[k];
+            delta = rng.uniform_real(-base, base);
+        }
+        last_delta = delta;
+        double c = std::cos(delta), sn = std::sin(delta);
+        std::vector<int> side = side_of(k);
+        mark(side, 1);
+        auto r = evaluate(side, side, T, [&] {
+            for (int i : side) rodrigues(&x[3 * i], o, u, c, sn);
+            return true;
+        });
+        mark(side, 0);
+        return r;
+    }
+
+    // try_crankshaft, src/physics_engine.cpp:2407-2501
+    std::pair<bool, bool> crankshaft(double T) {
+        last_kind = 1;
+        int ncp = t->n_cp;
+        if (ncp == 0) return {false, false};
+        int cp = ncp > 1 ? rng.uniform_int(ncp - 1) : 0;
+        last_bond = cp;
+        int pk = t->cp_phi[cp], sk = t->cp_psi[cp];
+        if (side_begin(pk) == side_end(pk) || side_begin(sk) == side_end(sk)) return {false, false};
+        std::vector<int> ps = side_of(pk), ss = side_of(sk);
+        mark(ps, 1);
+        double o1[3], u1[3];
+        if (!axis(pk, o1, u1)) {
+            mark(ps, 0);
+            return {false, false};
+        }
+        // old terms are taken before delta is drawn (src/physics_engine.cpp:2436-2444)
+        std::vector<double> op(3 * ps.size()), ob(ps.size());
+        for (size_t k = 0; k < ps.size(); ++k) {
+            int i = ps[k];
+            for (int d = 0; d < 3; ++d) op[3 * k + d] = x[3 * i + d];
+            ob[k] = a[i];
+        }
+        Items io, in;
+        Terms o = terms(want_items ? &io : nullptr);
+        double delta = rng.uniform_real(-cur_max, cur_max);
+        last_delta = delta;
+        double c = std::cos(delta), sn = std::sin(delta);
+        for (int i : ps) rodrigues(&x[3 * i], o1, u1, c, sn);
+        double o2[3], u2[3];
+        if (!axis(sk, o2, u2)) {
+            for (size_t k = 0; k < ps.size(); ++k) {
+                int i = ps[k];
+                for (int d = 0; d < 3; ++d) x[3 * i + d] = op[3 * k + d];
+            }
+            mark(ps, 0);
+            return {false, false};
+        }
+        for (int i : ss) rodrigues(&x[3 * i], o2, u2, c, -sn);
+        for (int i : ps) born_update(i, x.data(), s.rs, nl, a.data());
+        Terms nw = terms(want_items ? &in : nullptr);
+        double dE = (nw.cross - o.cross) + (nw.sasa - o.sasa) + (nw.dih - o.dih) + (nw.ss - o.ss);
+        last_dE = dE;
+        last_l1 = want_items ? l1_of(io, in) : 0.0;
+        bool acc = metropolis(dE, T);
+        if (acc)
+            curE += dE;
+        else
+            restore(ps, op, ob);
+        last_acc = acc;
+        mark(ps, 0);
+        return {acc, true};
+    }
+
+    // try_concerted_sidechain, src/physics_engine.cpp:2782-2878
+    std::pair<bool, bool> concerted(double T) {
+        last_kind = 2;
+        int ncsp = t->n_csp;
+        if (ncsp == 0) return {false, false};
+        int pi = ncsp > 1 ? rng.uniform_int(ncsp - 1) : 0;
+        last_bond = pi;
+        int ba = t->csp_a[pi], bb = t->csp_b[pi];
+        if (side_begin(ba) == side_end(ba) || side_begin(bb) == side_end(bb)) return {false, false};
+        double oA[3], uA[3], oB[3], uB[3];
+        if (!axis(ba, oA, uA)) return {false, false};
+        if (!axis(bb, oB, uB)) return {false, false};
+        double dA = cur_max * 2.5 * t->rb_scale[ba], dB = cur_max * 2.5 * t->rb_scale[bb];
+        double deltaA = rng.uniform_real(-dA, dA);
+        double deltaB = rng.uniform_real(-dB, dB);
+        last_delta = deltaA;
+        double cA = std::cos(deltaA), sA = std::sin(deltaA), cB = std::cos(deltaB), sB = std::sin(deltaB);
+        std::vector<int> sa = side_of(ba), sb = side_of(bb);
+        mark(sa, 1);
+        mark(sb, 2);
+        std::vector<int> all(sa);
+        all.insert(all.end(), sb.begin(), sb.end());
+        auto r = evaluate(all, all, T, [&] {
+            for (int i : sa) rodrigues(&x[3 * i], oA, uA, cA, sA);
+            for (int i : sb) rodrigues(&x[3 * i], oB, uB, cB, sB);
+            return true;
+        });
+        mark(all, 0);
+        return r;
+    }
+
+    void rebuild_if_needed() {
+        if (nl.stale(s.n, x.data())) {
+            nl.build(s.n, x.data());
+            born_full(s.n, x.data(), s.rs, nl, a.data(), nullptr);
+            curE = total_energy(s, x.data(), nl, a.data(), t);
+            ++rebuilds;
+        }
+    }
+
+    void tune(bool acc) {
+        // adaptive step, src/physics_engine.cpp:2519-2527
+        acc_win += acc ? 1 : 0;
+        if (++tot_win == 200) {
+            double rate = (double)acc_win / 200.0;
+            if (rate > 0.52)
+                cur_max = std::min(0.50, cur_max * 1.06);
+            else if (rate < 0.28)
+                cur_max = std::max(0.004, cur_max * 0.94);
+            acc_win = tot_win = 0;
+        }
+    }
+
+    void step(int mode, double T) {
+        rebuild_if_needed();
+        last_kind = -1;
+        last_acc = 0;
+        last_dE = 0;
+        last_l1 = 0;
+        std::pair<bool, bool> r;
+        if (mode == ORC_MODE_TORSION30) {
+            r = torsion(T, true);
+            if (r.second) ++moves;
+            if (r.first) ++accepted;
+            return;
+        }
+        if (mode == ORC_MODE_LANDSCAPE) {
+            // src/physics_engine.cpp:2899-2903
+            bool conc = t->n_csp > 0 && rng.canonical() < 0.15;
+            bool crank = !conc && t->n_cp > 0 && rng.canonical() < 0.25;
+            r = conc ? concerted(T) : crank ? crankshaft(T) : torsion(T, false);
+        } else {
+            bool crank = t->n_cp > 0 && rng.canonical() < 0.25;
+            r = crank ? crankshaft(T) : torsion(T, false);
+        }
+        if (r.second) {
+            ++moves;
+            tune(r.first);
+        }
+        if (r.first) ++accepted;
+    }
+};
+
+// ───────────────────────────── FP64 gradient ────────────────────────────────
+// No counterpart exists in the reference (it has no forces).  This is the
+// exact derivative of total_e (src/physics_engine.cpp:2205-2234) with the
+// neighbour list and the 12 A cutoff set held fixed, including the chain
+// rule through the HCT radii (unclamped atoms only) and the SASA fold.
+
+// d hct(r; ri, rj) / dr
+inline double dhct_dr(double r, double ri, double rj) {
+    double U = r + rj;
+    if (ri >= U) return 0.0;
+    double d = r - rj;
+    double L = std::max(std::abs(d), ri);
+    double dL = (std::abs(d) > ri) ? (d > 0 ? 1.0 : -1.0) : 0.0;
+    double c = ri * ri - rj * rj;
+    double B = (r * r + c) / (4.0 * ri * ri * r * r);
+    double dB = -2.0 * c / (4.0 * ri * ri * r * r * r);
+    double lg = std::log(L / U);
+    return -dL / (L * L) + 1.0 / (U * U) + dB * lg + B * (dL / L - 1.0 / U);
+}
+
+// pair energy derivatives: dE/dr and dE/dai, dE/daj
+inline void dpair(double r2, double qi, double qj, double ri, double rj, double ei, double ej,
+                  double ai, double aj, double* dEdr, double* dEdai, double* dEdaj) {
+    double r = std::sqrt(r2), sig = ri + rj, qp = qi * qj;
+    double edh = (COULOMB * qp) / (EPS_WATER * r) * std::exp(-KAPPA * r);
+    double dd = edh * (-1.0 / r - KAPPA);
+    double D = ai * aj, g = std::exp(-r2 / (4.0 * D));
+    double f = std::sqrt(r2 + D * g);
+    double egb = GB_COEF * qp / f;
+    double dfdr = (2.0 * r - r * g / 2.0) / (2.0 * f);
+    double dgb = -egb / f * dfdr;
+    double dfdD = g * (1.0 + r2 / (4.0 * D)) / (2.0 * f);
+    double eps = std::sqrt(ei * ej), s6 = std::pow(sig / r, 6);
+    double dlj = 4.0 * eps * (-12.0 * s6 * s6 + 6.0 * s6) / r;
+    double dhc = 0.0;
+    double rc = sig * HARD_FRAC;
+    if (r < rc) {
+        double p12 = std::pow(rc / r, 12.0);
+        double xx = p12 - 1.0;
+        if (HARD_SCALE * xx * xx < HARD_CAP) dhc = HARD_SCALE * 2.0 * xx * (-12.0 * p12 / r);
+    }
+    *dEdr = dd + dgb + dlj + dhc;
+    *dEdai = -egb / f * dfdD * aj;
+    *dEdaj = -egb / f * dfdD * ai;
+}
+
+inline void cross3(const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+// reverse-mode derivative of dih_angle w.r.t. the four atoms
+void ddih(const double* x, int a, int b, int c, int d, double gphi, double* G) {
+    double b1[3], b2[3], b3[3], n1[3], n2[3], m1[3];
+    for (int k = 0; k < 3; ++k) {
+        b1[k] = x[3 * a + k] - x[3 * b + k];
+        b2[k] = x[3 * c + k] - x[3 * b + k];
+        b3[k] = x[3 * d + k] - x[3 * c + k];
+    }
+    cross3(b1, b2, n1);
+    cross3(b2, b3, n2);
+    cross3(n1, b2, m1);
+    double X = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+    double Y = m1[0] * n2[0] + m1[1] * n2[1] + m1[2] * n2[2];
+    double den = X * X + Y * Y;
+    if (den == 0.0) return;
+    double gX = -Y / den * gphi, gY = X / den * gphi;
+    double gn1[3], gn2[3], gm1[3], gb1[3] = {0, 0, 0}, gb2[3] = {0, 0, 0}, gb3[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        gn1[k] = gX * n2[k];
+        gn2[k] = gX * n1[k] + gY * m1[k];
+        gm1[k] = gY * n2[k];
+    }
+    double t[3];
+    // m1 = n1 x b2
+    cross3(b2, gm1, t);
+    for (int k = 0; k < 3; ++k) gn1[k] += t[k];
+    cross3(gm1, n1, t);
+    for (int k = 0; k < 3; ++k) gb2[k] += t[k];
+    // n2 = b2 x b3
+    cross3(b3, gn2, t);
+    for (int k = 0; k < 3; ++k) gb2[k] += t[k];
+    cross3(gn2, b2, t);
+    for (int k = 0; k < 3; ++k) gb3[k] += t[k];
+    // n1 = b1 x b2
+    cross3(b2, gn1, t);
+    for (int k = 0; k < 3; ++k) gb1[k] += t[k];
+    cross3(gn1, b1, t);
+    for (int k = 0; k < 3; ++k) gb2[k] += t[k];
+    for (int k = 0; k < 3; ++k) {
+        G[3 * a + k] += gb1[k];
+        G[3 * b + k] += -gb1[k] - gb2[k];
+        G[3 * c + k] += gb2[k] - gb3[k];
+        G[3 * d + k] += gb3[k];
+    }
+}
+
+inline void add_radial(double* G, const double* x, int i, int j, double dEdr, double r) {
+    for (int k = 0; k < 3; ++k) {
+        double u = (x[3 * i + k] - x[3 * j + k]) / r;
+        G[3 * i + k] += dEdr * u;
+        G[3 * j + k] -= dEdr * u;
+    }
+}
+
+double gradient(const Sys& s, const double* x, const orc_topo* t, double* G) {
+    const int n = s.n;
+    std::fill(G, G + 3 * (size_t)n, 0.0);
+    NL nl;
+    nl.build(n, x);
+    std::vector<double> a(n), sum(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = nl.off[i]; k < nl.off[i + 1]; ++k) {
+            int j = nl.js[k];
+            double r2 = dist2(x, i, j), r = std::sqrt(r2);
+            sum[i] += hct(r, r2, s.rs[i], s.rs[j]);
+            sum[j] += hct(r, r2, s.rs[j], s.rs[i]);
+        }
+    std::vector<char> clamped(n);
+    for (int i = 0; i < n; ++i) {
+        double inv = 1.0 / s.rs[i] - 0.5 * sum[i];
+        clamped[i] = !(inv > 2.0);
+        a[i] = 1.0 / std::max(inv, 2.0);
+    }
+    double terms[ORC_NTERMS];
+    total_terms(s, x, nl, a.data(), t, terms);
+    // pair terms
+    std::vector<double> dEda(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = nl.off[i]; k < nl.off[i + 1]; ++k) {
+            int j = nl.js[k];
+            if (excluded(t, i, j)) continue;
+            double r2 = dist2(x, i, j);
+            if (r2 > PAIR_CUT2) continue;
+            double dr, dai, daj;
+            dpair(r2, s.q[i], s.q[j], s.rs[i], s.rs[j], s.ep[i], s.ep[j], a[i], a[j], &dr, &dai, &daj);
+            add_radial(G, x, i, j, dr, std::sqrt(r2));
+            dEda[i] += dai;
+            dEda[j] += daj;
+        }
+    // Born chain rule: a = 1/(1/R - S/2) when unclamped, da/dS = a^2/2
+    for (int i = 0; i < n; ++i)
+        for (int k = nl.off[i]; k < nl.off[i + 1]; ++k) {
+            int j = nl.js[k];
+            double wi = clamped[i] ? 0.0 : dEda[i] * 0.5 * a[i] * a[i];
+            double wj = clamped[j] ? 0.0 : dEda[j] * 0.5 * a[j] * a[j];
+            if (wi == 0.0 && wj == 0.0) continue;
+            double r = std::sqrt(dist2(x, i, j));
+            double d = wi * dhct_dr(r, s.rs[i], s.rs[j]) + wj * dhct_dr(r, s.rs[j], s.rs[i]);
+            add_radial(G, x, i, j, d, r);
+        }
+    // SASA fold adjoint
+    for (int i = 0; i < n; ++i) {
+        double ri = s.rs[i] + PROBE_R;
+        int b = nl.off[i], e = nl.off[i + 1];
+        std::vector<double> sa_before(e - b), rr(e - b);
+        std::vector<char> hit(e - b, 0), clampk(e - b, 0);
+        double sa = 4.0 * PI * ri * ri;
+        for (int k = b; k < e; ++k) {
+            int j = nl.js[k];
+            double rj = s.rs[j] + PROBE_R;
+            double r = std::sqrt(dist2(x, i, j)), dc = ri + rj;
+            rr[k - b] = r;
+            sa_before[k - b] = sa;
+            if (r >= dc) continue;
+            hit[k - b] = 1;
+            double cap = 2.0 * PI * ri * ri * ((dc - r) / (2.0 * ri));
+            if (sa * 0.85 <= cap) {
+                clampk[k - b] = 1;
+                sa -= sa * 0.85;
+            } else {
+                sa -= cap;
+            }
+        }
+        double lam = sa > 0.0 ? GAMMA_SA : 0.0;
+        for (int k = e - 1; k >= b; --k) {
+            if (!hit[k - b]) continue;
+            if (clampk[k - b]) {
+                lam *= 0.15;
+            } else {
+                // sa_after = sa_before - pi*ri*(dc - r)  =>  d/dr = +pi*ri
+                add_radial(G, x, i, nl.js[k], lam * PI * ri, rr[k - b]);
+            }
+        }
+    }
+    // dihedrals
+    if (t)
+        for (int k = 0; k < t->n_dih; ++k) {
+            int a4 = t->dih_a[k], b4 = t->dih_b[k], c4 = t->dih_c[k], d4 = t->dih_d[k];
+            double phi = dih_angle(x, a4, b4, c4, d4);
+            double g = 0.0;
+            for (int m = t->dih_toff[k]; m < t->dih_toff[k + 1]; ++m)
+                g += -t->dih_v2[m] * t->dih_n[m] * std::sin((double)t->dih_n[m] * phi - t->dih_gamma[m]);
+            ddih(x, a4, b4, c4, d4, g, G);
+        }
+    // disulfides
+    if (t)
+        for (int k = 0; k < t->n_ss; ++k) {
+            int i = t->ss_i[k], j = t->ss_j[k];
+            double r = std::sqrt(dist2(x, i, j));
+            add_radial(G, x, i, j, 2.0 * K_SS * (r - R0_SS), r);
+        }
+    return terms[ORC_TOTAL];
+}
+
+}  // namespace
+
+// ───────────────────────────────── C ABI ────────────────────────────────────
+extern "C" {
+
+int64_t orc_nlist(int n, const double* xyz, int* offs, int* js, int64_t js_cap) {
+    NL nl;
+    nl.build(n, xyz);
+    int64_t P = (int64_t)nl.js.size();
+    if (offs) std::copy(nl.off.begin(), nl.off.end(), offs);
+    if (js && js_cap >= P) std::copy(nl.js.begin(), nl.js.end(), js);
+    return P;
+}
+
+void orc_born(int n, const double* xyz, const double* rstar, const int* offs, const int* js,
+              double* a_out, double* half_sum_out) {
+    NL nl;
+    nl.off.assign(offs, offs + n + 1);
+    nl.js.assign(js, js + offs[n]);
+    born_full(n, xyz, rstar, nl, a_out, half_sum_out);
+}
+
+void orc_energy_terms(int n, const double* xyz, const double* q, const double* rstar,
+                      const double* eps, const orc_topo* topo, double* terms) {
+    for (int k = 0; k < ORC_NTERMS; ++k) terms[k] = 0.0;
+    if (n == 0) return;  // calculate_potential returns 0 for no particles
+    Sys s{n, q, rstar, eps};
+    NL nl;
+    nl.build(n, xyz);
+    std::vector<double> a(n);
+    born_full(n, xyz, rstar, nl, a.data(), nullptr);
+    total_terms(s, xyz, nl, a.data(), topo, terms);
+}
+
+double orc_pair_terms(const double* pi, const double* pj, double qi, double qj, double ri,
+                      double rj, double ei, double ej, double ai, double aj, double* split4) {
+    return pair_terms(pi, pj, qi, qj, ri, rj, ei, ej, ai, aj, split4);
+}
+
+double orc_dihedral_angle(const double* a, const double* b, const double* c, const double* d) {
+    double x[12];
+    for (int k = 0; k < 3; ++k) {
+        x[k] = a[k];
+        x[3 + k] = b[k];
+        x[6 + k] = c[k];
+        x[9 + k] = d[k];
+    }
+    return dih_angle(x, 0, 1, 2, 3);
+}
+
+void orc_mt_words(uint32_t seed, int64_t n, uint32_t* out) {
+    MT m(seed);
+    for (int64_t k = 0; k < n; ++k) out[k] = m();
+}
+
+int orc_mc_run(int n, double* xyz, const double* q, const double* rstar, const double* eps,
+               const orc_topo* topo, const orc_mc_params* p, orc_mc_stats* st, orc_mc_trace* tr,
+               double* snap_xyz, double* snap_e) {
+    if (n <= 0 || !topo || topo->n_rot <= 0 || p->steps <= 0) return -1;
+    Sys s{n, q, rstar, eps};
+    Chain ch(s, xyz, topo, p->seed, p->max_angle);
+    ch.want_items = tr && tr->dE_l1;
+    int snap = 0;
+    for (int k = 0; k < p->steps; ++k) {
+        int64_t before = ch.rng.used;
+        ch.step(p->mode, p->temperature);
+        if (tr) {
+            if (tr->kind) tr->kind[k] = ch.last_kind;
+            if (tr->bond) tr->bond[k] = ch.last_bond;
+            if (tr->delta) tr->delta[k] = ch.last_delta;
+            if (tr->dE) tr->dE[k] = ch.last_dE;
+            if (tr->dE_l1) tr->dE_l1[k] = ch.last_l1;
+            if (tr->accepted) tr->accepted[k] = ch.last_acc;
+            if (tr->words_before) tr->words_before[k] = before;
+        }
+        if (p->snapshot_every > 0 && (k + 1) % p->snapshot_every == 0) {
+            if (snap_xyz) std::copy(ch.x.begin(), ch.x.end(), snap_xyz + (size_t)snap * 3 * n);
+            if (snap_e) {
+                // fresh calculate_potential(st, &topo), src/physics_engine.cpp:2914-2920
+                double terms[ORC_NTERMS];
+                orc_energy_terms(n, ch.x.data(), q, rstar, eps, topo, terms);
+                snap_e[snap] = terms[ORC_TOTAL];
+            }
+            ++snap;
+        }
+    }
+    std::copy(ch.x.begin(), ch.x.end(), xyz);
+    if (st) {
+        st->cur_e = ch.curE;
+        st->cur_max = ch.cur_max;
+        st->accepted = ch.accepted;
+        st->moves = ch.moves;
+        st->rebuilds = ch.rebuilds;
+        st->words = ch.rng.used;
+    }
+    return 0;
+}
+
+int orc_mc_multi(int nchains, int n, const double* xyz0, const double* q, const double* rstar,
+                 const double* eps, const orc_topo* topo, const orc_mc_params* p, double* xyz_out,
+                 orc_mc_stats* st_out) {
+    if (n <= 0 || !topo || topo->n_rot <= 0 || p->steps <= 0 || nchains <= 0) return -1;
+    Sys s{n, q, rstar, eps};
+#pragma omp parallel for schedule(dynamic)
+    for (int c = 0; c < nchains; ++c) {
+        Chain ch(s, xyz0, topo, p->seed + (uint32_t)c, p->max_angle);
+        for (int k = 0; k < p->steps; ++k) ch.step(p->mode, p->temperature);
+        if (xyz_out) std::copy(ch.x.begin(), ch.x.end(), xyz_out + (size_t)c * 3 * n);
+        if (st_out) {
+            st_out[c].cur_e = ch.curE;
+            st_out[c].cur_max = ch.cur_max;
+            st_out[c].accepted = ch.accepted;
+            st_out[c].moves = ch.moves;
+            st_out[c].rebuilds = ch.rebuilds;
+            st_out[c].words = ch.rng.used;
+        }
+    }
+    return 0;
+}
+
+double orc_torsion_delta(int n, const double* xyz, const double* q, const double* rstar,
+                         const double* eps, const orc_topo* topo, const int* offs, const int* js,
+                         double* a, int bond, double delta, double* terms4, double* l1) {
+    Sys s{n, q, rstar, eps};
+    Chain ch(s, xyz, topo, 5489u, 0.12);
+    // use the caller's neighbour list and radii (state of a running chain)
+    ch.nl.off.assign(offs, offs + n + 1);
+    ch.nl.js.assign(js, js + offs[n]);
+    ch.a.assign(a, a + n);
+    double o[3], u[3];
+    if (!ch.axis(bond, o, u)) return 0.0;
+    std::vector<int> side = ch.side_of(bond);
+    ch.mark(side, 1);
+    Chain::Items io, in;
+    Chain::Terms old = ch.terms(&io);
+    double c = std::cos(delta), sn = std::sin(delta);
+    for (int i : side) rodrigues(&ch.x[3 * i], o, u, c, sn);
+    for (int i : side) born_update(i, ch.x.data(), s.rs, ch.nl, ch.a.data());
+    Chain::Terms nw = ch.terms(&in);
+    double L = Chain::l1_of(io, in);
+    if (terms4) {
+        terms4[0] = nw.cross - old.cross;
+        terms4[1] = nw.sasa - old.sasa;
+        terms4[2] = nw.dih - old.dih;
+        terms4[3] = nw.ss - old.ss;
+    }
+    if (l1) *l1 = L;
+    std::copy(ch.a.begin(), ch.a.end(), a);
+    return (nw.cross - old.cross) + (nw.sasa - old.sasa) + (nw.dih - old.dih) + (nw.ss - old.ss);
+}
+
+double orc_forces(int n, const double* xyz, const double* q, const double* rstar,
+                  const double* eps, const orc_topo* topo, double* dEdx, double* dEdtheta) {
+    Sys s{n, q, rstar, eps};
+    std::vector<double> G(3 * (size_t)n);
+    double E = gradient(s, xyz, topo, G.data());
+    if (dEdx) std::copy(G.begin(), G.end(), dEdx);
+    if (dEdtheta && topo) {
+        // rotating the j-side about the bond by dtheta moves atom k by
+        // u x (x_k - o) dtheta
+        for (int b = 0; b < topo->n_rot; ++b) {
+            int bi = topo->rb_i[b], bj = topo->rb_j[b];
+            double o[3], u[3];
+            for (int d = 0; d < 3; ++d) {
+                o[d] = xyz[3 * bi + d];
+                u[d] = xyz[3 * bj + d] - o[d];
+            }
+            double l = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            double acc = 0.0;
+            if (l > 1e-10) {
+                for (int d = 0; d < 3; ++d) u[d] /= l;
+                for (int m = topo->side_off[b]; m < topo->side_off[b + 1]; ++m) {
+                    int k = topo->side_idx[m];
+                    double v[3] = {xyz[3 * k] - o[0], xyz[3 * k + 1] - o[1], xyz[3 * k + 2] - o[2]}, w[3];
+                    cross3(u, v, w);
+                    acc += w[0] * G[3 * k] + w[1] * G[3 * k + 1] + w[2] * G[3 * k + 2];
+                }
+            }
+            dEdtheta[b] = acc;
+        }
+    }
+    return E;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+/*
+ * alma_oracle.h -- FP64 CPU restatement of ALMA's protein_physics energy and
+ * torsion-MC path.  TEST INFRASTRUCTURE ONLY: the checker that tests, the
+ * smoke entry point and the benchmark's CPU-baseline leg compare the HIP
+ * library against.  Nothing in the product links or calls this.
+ *
+ * Every function follows src/physics_engine.cpp of the reference; the .cpp
+ * cites file:line per function.
+ */
+#ifndef ALMA_ORACLE_H
+#define ALMA_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Topology in flat CSR form (the same arrays BondTopology exports,
+ * src/physics_engine.cpp:3746-3841, plus exclusions / sides / pairs). */
+typedef struct {
+    int n_atoms;
+    const int* excl_off;  const int* excl_idx;          /* sorted j > i        */
+    int n_rot;
+    const int* rb_i; const int* rb_j; const int* rb_kind; const double* rb_scale;
+    const int* side_off; const int* side_idx;           /* DFS order           */
+    int n_dih;
+    const int* dih_a; const int* dih_b; const int* dih_c; const int* dih_d;
+    const int* dih_toff; const double* dih_v2; const int* dih_n; const double* dih_gamma;
+    int n_ss;  const int* ss_i;  const int* ss_j;
+    int n_cp;  const int* cp_phi; const int* cp_psi;     /* concerted_pairs      */
+    int n_csp; const int* csp_a; const int* csp_b;       /* concerted sidechain  */
+} orc_topo;
+
+/* Per-term breakdown written by orc_energy_terms (kcal/mol). */
+enum { ORC_DH = 0, ORC_GB, ORC_LJ, ORC_HC, ORC_SASA, ORC_DIH, ORC_SS, ORC_TOTAL, ORC_NTERMS };
+
+/* Neighbor list in reference cell-scan order.  Returns number of pairs; fills
+ * offs[n+1] and js[] when js_cap >= pairs. */
+int64_t orc_nlist(int n, const double* xyz, int* offs, int* js, int64_t js_cap);
+/* Full-list HCT Born radii (two directions) and the half-list HCT sums. */
+void orc_born(int n, const double* xyz, const double* rstar, const int* offs,
+              const int* js, double* a_out, double* half_sum_out);
+/* calculate_potential with per-term split; topo may be NULL. */
+void orc_energy_terms(int n, const double* xyz, const double* q, const double* rstar,
+                      const double* eps, const orc_topo* topo, double* terms);
+/* Pair energy of one pair with its four-term split (DH, GB, LJ, HC). */
+double orc_pair_terms(const double* pi, const double* pj, double qi, double qj,
+                      double ri, double rj, double ei, double ej, double ai, double aj,
+                      double* split4);
+/* Reference dihedral angle (non-IUPAC atan2 form). */
+double orc_dihedral_angle(const double* a, const double* b, const double* c, const double* d);
+
+/* ---- RNG: std::mt19937 + libstdc++-11 distributions ---- */
+void orc_mt_words(uint32_t seed, int64_t n, uint32_t* out);
+
+/* ---- MC ---- */
+enum { ORC_MODE_ENSEMBLE = 0,   /* generate_ensemble chain semantics      */
+       ORC_MODE_LANDSCAPE = 1,  /* run_landscape_* semantics (concerted)   */
+       ORC_MODE_TORSION30 = 2   /* single torsion, delta ~ U[-pi/6, pi/6]  */ };
+
+typedef struct {
+    int mode;
+    int steps;              /* total MC steps                                  */
+    int snapshot_every;     /* landscape: steps per snapshot (0 = none)        */
+    double temperature;     /* kT in kcal/mol                                  */
+    double max_angle;       /* initial cur_max (rad)                           */
+    uint32_t seed;          /* mt19937 seed                                    */
+} orc_mc_params;
+
+typedef struct {
+    double cur_e;           /* running energy (curE)                           */
+    double cur_max;         /* final adaptive step                             */
+    int64_t accepted;
+    int64_t moves;          /* did_move steps                                  */
+    int64_t rebuilds;
+    int64_t words;          /* mt19937 words consumed                          */
+} orc_mc_stats;
+
+/* Per-step trace (optional, arrays of length steps; any may be NULL). */
+typedef struct {
+    int* kind;              /* 0 torsion, 1 crankshaft, 2 concerted, -1 none   */
+    int* bond;              /* rot bond / pair index                           */
+    double* delta;          /* proposed angle                                  */
+    double* dE;             /* Metropolis dE (FP64)                            */
+    double* dE_l1;          /* sum over terms of |per-item change|             */
+    int* accepted;
+    int64_t* words_before;  /* RNG cursor before the step                      */
+} orc_mc_trace;
+
+/* Run one chain.  xyz is updated in place.  snap_xyz (n_snap*3n) and
+ * snap_e (n_snap) receive landscape snapshots when snapshot_every > 0. */
+int orc_mc_run(int n, double* xyz, const double* q, const double* rstar, const double* eps,
+               const orc_topo* topo, const orc_mc_params* p, orc_mc_stats* st,
+               orc_mc_trace* tr, double* snap_xyz, double* snap_e);
+
+/* nchains independent chains (seed + c), OpenMP over chains as in
+ * generate_ensemble (src/physics_engine.cpp:2532-2541).  xyz_out: nchains*3n. */
+int orc_mc_multi(int nchains, int n, const double* xyz0, const double* q,
+                 const double* rstar, const double* eps, const orc_topo* topo,
+                 const orc_mc_params* p, double* xyz_out, orc_mc_stats* st_out);
+
+/* One Metropolis dE of a torsion move at the given state, reference
+ * semantics (cross pairs + full SASA difference + boundary dihedrals + side
+ * SS), with radii a[] (updated for side atoms when the function returns). */
+double orc_torsion_delta(int n, const double* xyz, const double* q, const double* rstar,
+                         const double* eps, const orc_topo* topo, const int* offs,
+                         const int* js, double* a, int bond, double delta,
+                         double* terms4 /* cross, sasa, dih, ss */, double* l1);
+
+/* FP64 analytic gradient of calculate_potential(particles, topo): dE/dx
+ * (3n) including the Born-radius chain rule, and dE/dtheta per rotatable bond. */
+double orc_forces(int n, const double* xyz, const double* q, const double* rstar,
+                  const double* eps, const orc_topo* topo, double* dEdx, double* dEdtheta);
+
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

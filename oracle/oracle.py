@@ -1,0 +1,277 @@
+"""ctypes wrapper of liboracle.so (FP64 CPU restatement of the reference
+protein_physics path).  TEST INFRASTRUCTURE: imported only by tests/, the
+smoke entry point and the benchmark's CPU-baseline leg, as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+TERMS = ("DH", "GB", "LJ", "HC", "SASA", "DIH", "SS", "TOTAL")
+MODE_ENSEMBLE, MODE_LANDSCAPE, MODE_TORSION30 = 0, 1, 2
+
+_I = C.POINTER(C.c_int)
+_D = C.POINTER(C.c_double)
+
+
+class Topo(C.Structure):
+    _fields_ = [("n_atoms", C.c_int), ("excl_off", _I), ("excl_idx", _I),
+                ("n_rot", C.c_int), ("rb_i", _I), ("rb_j", _I), ("rb_kind", _I),
+                ("rb_scale", _D), ("side_off", _I), ("side_idx", _I),
+                ("n_dih", C.c_int), ("dih_a", _I), ("dih_b", _I), ("dih_c", _I),
+                ("dih_d", _I), ("dih_toff", _I), ("dih_v2", _D), ("dih_n", _I),
+                ("dih_gamma", _D), ("n_ss", C.c_int), ("ss_i", _I), ("ss_j", _I),
+                ("n_cp", C.c_int), ("cp_phi", _I), ("cp_psi", _I),
+                ("n_csp", C.c_int), ("csp_a", _I), ("csp_b", _I)]
+
+
+class McParams(C.Structure):
+    _fields_ = [("mode", C.c_int), ("steps", C.c_int), ("snapshot_every", C.c_int),
+                ("temperature", C.c_double), ("max_angle", C.c_double),
+                ("seed", C.c_uint32)]
+
+
+class McStats(C.Structure):
+    _fields_ = [("cur_e", C.c_double), ("cur_max", C.c_double), ("accepted", C.c_int64),
+                ("moves", C.c_int64), ("rebuilds", C.c_int64), ("words", C.c_int64)]
+
+
+class McTrace(C.Structure):
+    _fields_ = [("kind", _I), ("bond", _I), ("delta", _D), ("dE", _D), ("dE_l1", _D),
+                ("accepted", _I), ("words_before", C.POINTER(C.c_int64))]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = C.CDLL(path)
+        L.orc_nlist.restype = C.c_int64
+        L.orc_nlist.argtypes = [C.c_int, _D, _I, _I, C.c_int64]
+        L.orc_born.argtypes = [C.c_int, _D, _D, _I, _I, _D, _D]
+        L.orc_energy_terms.argtypes = [C.c_int, _D, _D, _D, _D, C.POINTER(Topo), _D]
+        L.orc_pair_terms.restype = C.c_double
+        L.orc_pair_terms.argtypes = [_D, _D] + [C.c_double] * 8 + [_D]
+        L.orc_dihedral_angle.restype = C.c_double
+        L.orc_dihedral_angle.argtypes = [_D, _D, _D, _D]
+        L.orc_mt_words.argtypes = [C.c_uint32, C.c_int64, C.POINTER(C.c_uint32)]
+        L.orc_mc_run.restype = C.c_int
+        L.orc_mc_run.argtypes = [C.c_int, _D, _D, _D, _D, C.POINTER(Topo), C.POINTER(McParams),
+                                 C.POINTER(McStats), C.POINTER(McTrace), _D, _D]
+        L.orc_mc_multi.restype = C.c_int
+        L.orc_mc_multi.argtypes = [C.c_int, C.c_int, _D, _D, _D, _D, C.POINTER(Topo),
+                                   C.POINTER(McParams), _D, C.POINTER(McStats)]
+        L.orc_torsion_delta.restype = C.c_double
+        L.orc_torsion_delta.argtypes = [C.c_int, _D, _D, _D, _D, C.POINTER(Topo), _I, _I, _D,
+                                        C.c_int, C.c_double, _D, _D]
+        L.orc_forces.restype = C.c_double
+        L.orc_forces.argtypes = [C.c_int, _D, _D, _D, _D, C.POINTER(Topo), _D, _D]
+        L.orc_num_threads.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class TopoHandle:
+    """Keeps the numpy arrays alive behind an orc_topo struct."""
+
+    def __init__(self, csr: dict, n_atoms: int):
+        self.arrs = {}
+        t = Topo()
+        t.n_atoms = n_atoms
+        for k in ("excl_off", "excl_idx", "rb_i", "rb_j", "rb_kind", "side_off", "side_idx",
+                  "dih_a", "dih_b", "dih_c", "dih_d", "dih_toff", "dih_n", "ss_i", "ss_j",
+                  "cp_phi", "cp_psi", "csp_a", "csp_b"):
+            arr = _i32(csr.get(k, np.zeros(1, np.int32)))
+            if arr.size == 0:
+                arr = np.zeros(1, np.int32)
+            self.arrs[k] = arr
+            setattr(t, k, _p(arr, C.c_int))
+        for k in ("rb_scale", "dih_v2", "dih_gamma"):
+            arr = _f64(csr.get(k, np.zeros(1)))
+            if arr.size == 0:
+                arr = np.zeros(1)
+            self.arrs[k] = arr
+            setattr(t, k, _p(arr, C.c_double))
+        t.n_rot = len(csr.get("rb_i", []))
+        t.n_dih = len(csr.get("dih_a", []))
+        t.n_ss = len(csr.get("ss_i", []))
+        t.n_cp = len(csr.get("cp_phi", []))
+        t.n_csp = len(csr.get("csp_a", []))
+        self.struct = t
+
+    def ptr(self):
+        return C.byref(self.struct)
+
+
+def nlist(xyz):
+    xyz = _f64(xyz)
+    n = len(xyz)
+    offs = np.zeros(n + 1, np.int32)
+    P = lib().orc_nlist(n, _p(xyz, C.c_double), _p(offs, C.c_int), None, 0)
+    js = np.zeros(max(P, 1), np.int32)
+    lib().orc_nlist(n, _p(xyz, C.c_double), _p(offs, C.c_int), _p(js, C.c_int), P)
+    return offs, js[:P]
+
+
+def born(xyz, rstar, offs, js):
+    xyz = _f64(xyz)
+    n = len(xyz)
+    a = np.zeros(n)
+    half = np.zeros(n)
+    lib().orc_born(n, _p(xyz, C.c_double), _p(_f64(rstar), C.c_double), _p(_i32(offs), C.c_int),
+                   _p(_i32(js), C.c_int), _p(a, C.c_double), _p(half, C.c_double))
+    return a, half
+
+
+def energy_terms(sysd, topo_csr=None):
+    """dict term -> value, calculate_potential semantics (topo may be None)."""
+    xyz = _f64(sysd["xyz"])
+    n = len(xyz)
+    th = TopoHandle(topo_csr, n) if topo_csr is not None else None
+    out = np.zeros(8)
+    lib().orc_energy_terms(n, _p(xyz, C.c_double), _p(_f64(sysd["charge"]), C.c_double),
+                           _p(_f64(sysd["rstar"]), C.c_double), _p(_f64(sysd["eps"]), C.c_double),
+                           th.ptr() if th else None, _p(out, C.c_double))
+    return dict(zip(TERMS, out.tolist()))
+
+
+def mt_words(seed, n):
+    out = np.zeros(n, np.uint32)
+    lib().orc_mt_words(seed, n, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def mc_run(sysd, topo_csr, steps, temperature=0.6, max_angle=0.12, seed=5489,
+           mode=MODE_ENSEMBLE, snapshot_every=0, trace=False, l1=False):
+    xyz = _f64(sysd["xyz"]).copy()
+    n = len(xyz)
+    th = TopoHandle(topo_csr, n)
+    p = McParams(mode, steps, snapshot_every, temperature, max_angle, seed)
+    st = McStats()
+    tr = None
+    arrays = {}
+    if trace:
+        arrays = {"kind": np.zeros(steps, np.int32), "bond": np.zeros(steps, np.int32),
+                  "delta": np.zeros(steps), "dE": np.zeros(steps),
+                  "dE_l1": np.zeros(steps) if l1 else None,
+                  "accepted": np.zeros(steps, np.int32),
+                  "words_before": np.zeros(steps, np.int64)}
+        tr = McTrace(_p(arrays["kind"], C.c_int), _p(arrays["bond"], C.c_int),
+                     _p(arrays["delta"], C.c_double), _p(arrays["dE"], C.c_double),
+                     _p(arrays["dE_l1"], C.c_double), _p(arrays["accepted"], C.c_int),
+                     arrays["words_before"].ctypes.data_as(C.POINTER(C.c_int64)))
+    nsnap = steps // snapshot_every if snapshot_every else 0
+    snap_x = np.zeros((max(nsnap, 1), n, 3))
+    snap_e = np.zeros(max(nsnap, 1))
+    rc = lib().orc_mc_run(n, _p(xyz, C.c_double), _p(_f64(sysd["charge"]), C.c_double),
+                          _p(_f64(sysd["rstar"]), C.c_double), _p(_f64(sysd["eps"]), C.c_double),
+                          th.ptr(), C.byref(p), C.byref(st), C.byref(tr) if tr else None,
+                          _p(snap_x, C.c_double), _p(snap_e, C.c_double))
+    if rc != 0:
+        raise ValueError("orc_mc_run rejected its arguments")
+    stats = {f: getattr(st, f) for f, _ in McStats._fields_}
+    res = {"xyz": xyz, "stats": stats}
+    if trace:
+        res["trace"] = {k: v for k, v in arrays.items() if v is not None}
+    if nsnap:
+        res["snapshots"] = snap_x[:nsnap]
+        res["energies"] = snap_e[:nsnap]
+    return res
+
+
+def mc_multi(sysd, topo_csr, nchains, steps, temperature=0.6, max_angle=0.12, seed=5489,
+             mode=MODE_ENSEMBLE):
+    xyz0 = _f64(sysd["xyz"])
+    n = len(xyz0)
+    th = TopoHandle(topo_csr, n)
+    p = McParams(mode, steps, 0, temperature, max_angle, seed)
+    out = np.zeros((nchains, n, 3))
+    st = (McStats * nchains)()
+    rc = lib().orc_mc_multi(nchains, n, _p(xyz0, C.c_double), _p(_f64(sysd["charge"]), C.c_double),
+                            _p(_f64(sysd["rstar"]), C.c_double), _p(_f64(sysd["eps"]), C.c_double),
+                            th.ptr(), C.byref(p), _p(out, C.c_double), st)
+    if rc != 0:
+        raise ValueError("orc_mc_multi rejected its arguments")
+    return out, [{f: getattr(s, f) for f, _ in McStats._fields_} for s in st]
+
+
+def torsion_delta(sysd, topo_csr, offs, js, a, bond, delta):
+    xyz = _f64(sysd["xyz"])
+    n = len(xyz)
+    th = TopoHandle(topo_csr, n)
+    a = _f64(a).copy()
+    t4 = np.zeros(4)
+    l1 = np.zeros(1)
+    dE = lib().orc_torsion_delta(n, _p(xyz, C.c_double), _p(_f64(sysd["charge"]), C.c_double),
+                                 _p(_f64(sysd["rstar"]), C.c_double), _p(_f64(sysd["eps"]), C.c_double),
+                                 th.ptr(), _p(_i32(offs), C.c_int), _p(_i32(js), C.c_int),
+                                 _p(a, C.c_double), bond, delta, _p(t4, C.c_double), _p(l1, C.c_double))
+    return dE, t4, float(l1[0]), a
+
+
+def forces(sysd, topo_csr):
+    xyz = _f64(sysd["xyz"])
+    n = len(xyz)
+    th = TopoHandle(topo_csr, n) if topo_csr is not None else None
+    g = np.zeros((n, 3))
+    nrot = len(topo_csr["rb_i"]) if topo_csr is not None else 0
+    gt = np.zeros(max(nrot, 1))
+    E = lib().orc_forces(n, _p(xyz, C.c_double), _p(_f64(sysd["charge"]), C.c_double),
+                         _p(_f64(sysd["rstar"]), C.c_double), _p(_f64(sysd["eps"]), C.c_double),
+                         th.ptr() if th else None, _p(g, C.c_double), _p(gt, C.c_double))
+    return E, g, gt[:nrot]
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def topo_to_csr(t) -> dict:
+    """Flatten any object exposing the reference BondTopology protocol
+    (src/physics_engine.cpp:3729-3841) into CSR arrays."""
+    excl = t.excl
+    excl_off = np.zeros(len(excl) + 1, np.int32)
+    excl_off[1:] = np.cumsum([len(v) for v in excl])
+    excl_idx = np.array([j for v in excl for j in v], np.int32)
+    sides = t.rot_bond_sides
+    side_off = np.zeros(len(sides) + 1, np.int32)
+    side_off[1:] = np.cumsum([len(v) for v in sides])
+    side_idx = np.array([j for v in sides for j in v], np.int32)
+    ss = list(t.disulfide_pairs)
+    cp = list(t.concerted_pairs)
+    csp = list(t.concerted_sidechain_pairs)
+    return {
+        "excl_off": excl_off, "excl_idx": excl_idx,
+        "rb_i": np.array(t.rb_atom_i, np.int32), "rb_j": np.array(t.rb_atom_j, np.int32),
+        "rb_kind": np.array(t.rb_kind, np.int32),
+        "rb_scale": np.array(t.rot_bond_scale, np.float64),
+        "side_off": side_off, "side_idx": side_idx,
+        "dih_a": np.array(t.dih_a, np.int32), "dih_b": np.array(t.dih_b, np.int32),
+        "dih_c": np.array(t.dih_c, np.int32), "dih_d": np.array(t.dih_d, np.int32),
+        "dih_toff": np.array(t.dih_term_offsets, np.int32),
+        "dih_v2": np.array(t.dih_term_v2, np.float64),
+        "dih_n": np.array(t.dih_term_n, np.int32),
+        "dih_gamma": np.array(t.dih_term_gamma, np.float64),
+        "ss_i": np.array([p[0] for p in ss], np.int32), "ss_j": np.array([p[1] for p in ss], np.int32),
+        "cp_phi": np.array([p[0] for p in cp], np.int32), "cp_psi": np.array([p[1] for p in cp], np.int32),
+        "csp_a": np.array([p[0] for p in csp], np.int32), "csp_b": np.array([p[1] for p in csp], np.int32),
+    }

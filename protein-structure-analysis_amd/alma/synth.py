@@ -1,0 +1,877 @@
+"""Seeded synthetic proteins for the energy / MC workloads.
+
+A system is a dict of numpy arrays::
+
+    xyz (N,3) f64, charge (N,) f64, rstar (N,) f64, eps (N,) f64,
+    resnames (N,) list[str], atomnames (N,) list[str], res_idx (N,) int32,
+    ss_pairs (K,2) int32 (SG-SG pairs to register with add_disulfide)
+
+which is exactly what ALMA's ``_parse_pdb`` hands to ``Particle`` and
+``BondTopology.build`` (python/gui_main.py:518-590).
+
+Generation:
+  * residues uniform over the 20 amino acids (HIS stored as HID);
+  * ideal internal geometry with all hydrogens, omega = 180 deg;
+  * backbone phi/psi: 60 % helix basin N(-63, 10) / N(-42, 10),
+    40 % sheet basin N(-120, 15) / N(130, 15) (degrees);
+  * chi angles: a rotamer well chosen uniformly (sp3-sp3: -60/180/60,
+    sp3-sp2 ends: +-90), plus N(0, 8 deg) jitter;
+  * charges / R* / epsilon from the ff14SB table (``ff14sb``);
+  * clash-free chain growth (backtracking) inside a sphere of radius
+    2.0 * Rg_target about the first CA, then
+  * compaction to Rg(CA) <= 2.2 * n_res**0.38 A by pivot moves on backbone
+    phi/psi (new angle redrawn from the basins), each accepted only if it
+    does not increase Rg beyond a small Metropolis allowance and brings no
+    backbone/CB atom within 2.8 A of the rest of the chain (reduced model,
+    tracked in torsion space); the all-atom model is then built once from
+    the final torsions and rotamers of residues left in a clash are redrawn.
+    Whole-chain rejection on Rg essentially never succeeds at 300 residues,
+    so pivot compaction is the generator's rejection step.
+
+Everything is driven by ``numpy.random.Generator(PCG64(seed))`` so a seed
+fixes the system bit-for-bit on a given numpy build.
+"""
+from __future__ import annotations
+
+import math
+import numpy as np
+
+from .ff14sb import RESIDUES, STANDARD_20, atom_params
+
+# ───────────────────────────── geometry helpers ─────────────────────────────
+
+
+def place(a, b, c, bond, angle_deg, tors_deg):
+    """Point D with |CD| = bond, angle(B,C,D) = angle, dihedral(A,B,C,D) = tors
+    (IUPAC sign), by the natural-extension reference frame construction."""
+    ang = math.radians(angle_deg)
+    tor = math.radians(tors_deg)
+    bc = c - b
+    bc /= np.linalg.norm(bc)
+    n = np.cross(b - a, bc)
+    n /= np.linalg.norm(n)
+    m = np.cross(n, bc)
+    d2 = np.array([-bond * math.cos(ang),
+                   bond * math.sin(ang) * math.cos(tor),
+                   bond * math.sin(ang) * math.sin(tor)])
+    return c + d2[0] * bc + d2[1] * m + d2[2] * n
+
+
+def dihedral_iupac(a, b, c, d):
+    b0 = a - b
+    b1 = c - b
+    b2 = d - c
+    b1n = b1 / np.linalg.norm(b1)
+    v = b0 - np.dot(b0, b1n) * b1n
+    w = b2 - np.dot(b2, b1n) * b1n
+    x = np.dot(v, w)
+    y = np.dot(np.cross(b1n, v), w)
+    return math.degrees(math.atan2(y, x))
+
+
+def _rotation(axis, theta):
+    axis = axis / np.linalg.norm(axis)
+    k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]],
+                  [-axis[1], axis[0], 0]])
+    return np.eye(3) + math.sin(theta) * k + (1 - math.cos(theta)) * (k @ k)
+
+
+# ─────────────────────────── residue construction ───────────────────────────
+# Side-chain heavy atoms: (atom, parent, a2, a3, bond, angle(a2,parent,atom),
+# torsion) where torsion is a float or ("chiK", offset).  Ideal values.
+_SC = {
+    "ALA": [], "GLY": [],
+    "SER": [("OG", "CB", "CA", "N", 1.417, 110.8, ("chi1", 0.0))],
+    "CYS": [("SG", "CB", "CA", "N", 1.808, 113.8, ("chi1", 0.0))],
+    "VAL": [("CG1", "CB", "CA", "N", 1.527, 110.7, ("chi1", 0.0)),
+            ("CG2", "CB", "CA", "N", 1.527, 110.4, ("chi1", 122.9))],
+    "THR": [("OG1", "CB", "CA", "N", 1.430, 109.2, ("chi1", 0.0)),
+            ("CG2", "CB", "CA", "N", 1.530, 111.1, ("chi1", -120.0))],
+    "LEU": [("CG", "CB", "CA", "N", 1.530, 116.1, ("chi1", 0.0)),
+            ("CD1", "CG", "CB", "CA", 1.520, 110.3, ("chi2", 0.0)),
+            ("CD2", "CG", "CB", "CA", 1.520, 110.6, ("chi2", 122.5))],
+    "ILE": [("CG1", "CB", "CA", "N", 1.530, 110.4, ("chi1", 0.0)),
+            ("CG2", "CB", "CA", "N", 1.530, 110.5, ("chi1", -123.0)),
+            ("CD1", "CG1", "CB", "CA", 1.520, 113.9, ("chi2", 0.0))],
+    "MET": [("CG", "CB", "CA", "N", 1.520, 114.1, ("chi1", 0.0)),
+            ("SD", "CG", "CB", "CA", 1.810, 112.7, ("chi2", 0.0)),
+            ("CE", "SD", "CG", "CB", 1.790, 100.6, ("chi3", 0.0))],
+    "PRO": [],   # ring atoms CD, CG are closed onto N by _proline_ring
+    "PHE": [("CG", "CB", "CA", "N", 1.500, 113.8, ("chi1", 0.0)),
+            ("CD1", "CG", "CB", "CA", 1.390, 120.0, ("chi2", 0.0)),
+            ("CD2", "CG", "CB", "CA", 1.390, 120.0, ("chi2", 180.0)),
+            ("CE1", "CD1", "CG", "CB", 1.390, 120.0, 180.0),
+            ("CE2", "CD2", "CG", "CB", 1.390, 120.0, 180.0),
+            ("CZ", "CE1", "CD1", "CG", 1.390, 120.0, 0.0)],
+    "TYR": [("CG", "CB", "CA", "N", 1.510, 113.9, ("chi1", 0.0)),
+            ("CD1", "CG", "CB", "CA", 1.390, 120.8, ("chi2", 0.0)),
+            ("CD2", "CG", "CB", "CA", 1.390, 120.8, ("chi2", 180.0)),
+            ("CE1", "CD1", "CG", "CB", 1.390, 121.2, 180.0),
+            ("CE2", "CD2", "CG", "CB", 1.390, 121.2, 180.0),
+            ("CZ", "CE1", "CD1", "CG", 1.390, 119.6, 0.0),
+            ("OH", "CZ", "CE1", "CD1", 1.360, 119.9, 180.0)],
+    "TRP": [("CG", "CB", "CA", "N", 1.500, 114.1, ("chi1", 0.0)),
+            ("CD1", "CG", "CB", "CA", 1.370, 127.0, ("chi2", 0.0)),
+            ("CD2", "CG", "CB", "CA", 1.430, 126.6, ("chi2", 180.0)),
+            ("NE1", "CD1", "CG", "CB", 1.380, 108.5, 180.0),
+            ("CE2", "CD2", "CG", "CB", 1.400, 107.2, 180.0),
+            ("CE3", "CD2", "CG", "CB", 1.400, 133.9, 0.0),
+            ("CZ2", "CE2", "CD2", "CG", 1.400, 122.4, 180.0),
+            ("CZ3", "CE3", "CD2", "CG", 1.390, 118.7, 180.0),
+            ("CH2", "CZ2", "CE2", "CD2", 1.370, 117.5, 0.0)],
+    "HIS": [("CG", "CB", "CA", "N", 1.500, 113.7, ("chi1", 0.0)),
+            ("ND1", "CG", "CB", "CA", 1.380, 122.7, ("chi2", 0.0)),
+            ("CD2", "CG", "CB", "CA", 1.360, 131.0, ("chi2", 180.0)),
+            ("CE1", "ND1", "CG", "CB", 1.320, 109.0, 180.0),
+            ("NE2", "CD2", "CG", "CB", 1.370, 107.0, 180.0)],
+    "ASP": [("CG", "CB", "CA", "N", 1.520, 113.0, ("chi1", 0.0)),
+            ("OD1", "CG", "CB", "CA", 1.250, 119.2, ("chi2", 0.0)),
+            ("OD2", "CG", "CB", "CA", 1.250, 118.2, ("chi2", 180.0))],
+    "ASN": [("CG", "CB", "CA", "N", 1.520, 112.6, ("chi1", 0.0)),
+            ("OD1", "CG", "CB", "CA", 1.230, 120.8, ("chi2", 0.0)),
+            ("ND2", "CG", "CB", "CA", 1.330, 116.4, ("chi2", 180.0))],
+    "GLU": [("CG", "CB", "CA", "N", 1.520, 114.0, ("chi1", 0.0)),
+            ("CD", "CG", "CB", "CA", 1.520, 113.3, ("chi2", 0.0)),
+            ("OE1", "CD", "CG", "CB", 1.250, 119.0, ("chi3", 0.0)),
+            ("OE2", "CD", "CG", "CB", 1.250, 118.1, ("chi3", 180.0))],
+    "GLN": [("CG", "CB", "CA", "N", 1.520, 114.0, ("chi1", 0.0)),
+            ("CD", "CG", "CB", "CA", 1.520, 112.8, ("chi2", 0.0)),
+            ("OE1", "CD", "CG", "CB", 1.230, 121.0, ("chi3", 0.0)),
+            ("NE2", "CD", "CG", "CB", 1.330, 116.5, ("chi3", 180.0))],
+    "LYS": [("CG", "CB", "CA", "N", 1.520, 113.8, ("chi1", 0.0)),
+            ("CD", "CG", "CB", "CA", 1.520, 111.8, ("chi2", 0.0)),
+            ("CE", "CD", "CG", "CB", 1.520, 111.7, ("chi3", 0.0)),
+            ("NZ", "CE", "CD", "CG", 1.490, 111.9, ("chi4", 0.0))],
+    "ARG": [("CG", "CB", "CA", "N", 1.520, 113.8, ("chi1", 0.0)),
+            ("CD", "CG", "CB", "CA", 1.520, 111.3, ("chi2", 0.0)),
+            ("NE", "CD", "CG", "CB", 1.460, 112.0, ("chi3", 0.0)),
+            ("CZ", "NE", "CD", "CG", 1.330, 124.2, ("chi4", 0.0)),
+            ("NH1", "CZ", "NE", "CD", 1.330, 120.0, 0.0),
+            ("NH2", "CZ", "NE", "CD", 1.330, 120.0, 180.0)],
+}
+# chi wells per residue: "sp3" -> (-60, 180, 60); "sp2" -> (-90, 90);
+# "gd" -> (-90, 90, 180)
+_CHI_KIND = {
+    "SER": ["sp3"], "CYS": ["sp3"], "VAL": ["sp3"], "THR": ["sp3"],
+    "LEU": ["sp3", "sp3"], "ILE": ["sp3", "sp3"], "MET": ["sp3", "sp3", "sp3"],
+    "PHE": ["sp3", "sp2"], "TYR": ["sp3", "sp2"], "TRP": ["sp3", "sp2"],
+    "HIS": ["sp3", "sp2"], "ASP": ["sp3", "sp2"], "ASN": ["sp3", "sp2"],
+    "GLU": ["sp3", "sp3", "sp2"], "GLN": ["sp3", "sp3", "sp2"],
+    "LYS": ["sp3", "sp3", "sp3", "sp3"], "ARG": ["sp3", "sp3", "sp3", "gd"],
+}
+_WELLS = {"sp3": (-60.0, 180.0, 60.0), "sp2": (-90.0, 90.0), "gd": (-90.0, 90.0, 180.0)}
+# ring closures (heavy-heavy bonds not given by the z-matrix parent links)
+_RING = {"PHE": [("CZ", "CE2")], "TYR": [("CZ", "CE2")],
+         "TRP": [("NE1", "CE2"), ("CZ3", "CH2")], "HIS": [("CE1", "NE2")],
+         "PRO": [("CD", "N"), ("CB", "CG"), ("CG", "CD")]}
+# heavy atoms whose hydrogens lie in the plane of their neighbours
+_SP2 = {"PHE": {"CD1", "CD2", "CE1", "CE2", "CZ"},
+        "TYR": {"CD1", "CD2", "CE1", "CE2"},
+        "TRP": {"CD1", "NE1", "CZ2", "CH2", "CZ3", "CE3"},
+        "HIS": {"ND1", "CE1", "CD2"}, "ASN": {"ND2"}, "GLN": {"NE2"},
+        "ARG": {"NE", "NH1", "NH2"}}
+_XH = {"C": 1.09, "N": 1.01, "O": 0.96, "S": 1.34}
+
+
+def _sample_chis(rng, res):
+    out = {}
+    for k, kind in enumerate(_CHI_KIND.get(res, [])):
+        wells = _WELLS[kind]
+        out[f"chi{k + 1}"] = wells[int(rng.integers(len(wells)))] + rng.normal(0.0, 8.0)
+    return out
+
+
+def _sample_phipsi(rng):
+    if rng.random() < 0.6:
+        return rng.normal(-63.0, 10.0), rng.normal(-42.0, 10.0)
+    return rng.normal(-120.0, 15.0), rng.normal(130.0, 15.0)
+
+
+def _residue_atoms(res, nterm, cterm):
+    """Storage-order atom list (name, type-charge record) for one residue."""
+    rec = list(RESIDUES[res])
+    names = [a for a, _, _ in rec]
+    if nterm:
+        i = names.index("N")
+        # drop backbone H, add H1 H2 H3 right after N
+        names = [a for a in names if a != "H"]
+        names = names[:i + 1] + ["H1", "H2", "H3"] + names[i + 1:]
+    if cterm:
+        names.append("OXT")
+    return names
+
+
+def _build_chain(rng, seq, phis, psis, chis):
+    """Cartesian coordinates for one chain (storage order) from torsions."""
+    n = len(seq)
+    pos = []           # per residue dict name -> xyz
+    N0 = np.array([0.0, 0.0, 0.0])
+    CA0 = np.array([1.458, 0.0, 0.0])
+    ang = math.radians(111.2)
+    C0 = CA0 + 1.525 * np.array([-math.cos(ang), math.sin(ang), 0.0])
+    prev = None
+    for r in range(n):
+        res = seq[r]
+        p = {}
+        if r == 0:
+            p["N"], p["CA"], p["C"] = N0, CA0, C0
+            # put C so that phi of residue 0 is irrelevant (no C_prev)
+        else:
+            p["N"] = place(prev["N"], prev["CA"], prev["C"], 1.329, 116.2, psis[r - 1])
+            p["CA"] = place(prev["CA"], prev["C"], p["N"], 1.458, 121.7, 180.0)
+            p["C"] = place(prev["C"], p["N"], p["CA"], 1.525, 111.2, phis[r])
+        # carbonyl O (and OXT) from psi
+        if r + 1 < n:
+            p["O"] = place(p["N"], p["CA"], p["C"], 1.231, 120.5, psis[r] + 180.0)
+        else:
+            p["O"] = place(p["N"], p["CA"], p["C"], 1.250, 117.0, psis[r] + 180.0)
+            p["OXT"] = place(p["N"], p["CA"], p["C"], 1.250, 117.0, psis[r])
+        if res != "GLY":
+            p["CB"] = place(p["C"], p["N"], p["CA"], 1.530, 110.5, -122.7)
+        for atom, par, a2, a3, bond, angle, tors in _SC[res]:
+            if isinstance(tors, tuple):
+                tors = chis[r][tors[0]] + tors[1]
+            p[atom] = place(p[a3], p[a2], p[par], bond, angle, tors)
+        if res == "PRO":
+            _proline_ring(p, phis[r])
+        pos.append(p)
+        prev = p
+
+    # hydrogens, parent = nearest preceding heavy atom in storage order
+    coords = []
+    names_all = []
+    res_of = []
+    bonds = []
+    prev_c = None
+    for r in range(n):
+        res = seq[r]
+        names = _residue_atoms(res, r == 0, r == n - 1)
+        p = pos[r]
+        heavy_nb = _heavy_neighbours(res, p, r == 0, r == n - 1,
+                                     pos[r - 1] if r > 0 else None,
+                                     pos[r + 1] if r + 1 < n else None)
+        parent = None
+        hs_of = {}
+        for nm in names:
+            if nm[0] != "H":
+                parent = nm
+            else:
+                hs_of.setdefault(parent, []).append(nm)
+        for par, hs in hs_of.items():
+            _place_h(res, par, hs, p, heavy_nb, nterm=(r == 0))
+        gidx = {nm: len(coords) + k for k, nm in enumerate(names)}
+        for a, lst in heavy_nb.items():
+            for b, _ in lst:
+                if b in gidx and gidx[a] < gidx[b]:
+                    bonds.append((gidx[a], gidx[b]))
+        for par, hs in hs_of.items():
+            for h in hs:
+                bonds.append((gidx[par], gidx[h]))
+        if prev_c is not None:
+            bonds.append((prev_c, gidx["N"]))
+        prev_c = gidx["C"]
+        for nm in names:
+            coords.append(p[nm])
+            names_all.append(nm)
+            res_of.append(r)
+    return np.array(coords), names_all, np.array(res_of), bonds
+
+
+def _heavy_neighbours(res, p, nterm, cterm, prevp, nextp):
+    """name -> list of neighbouring heavy-atom positions (incl. peptide links)."""
+    bonds = [("N", "CA"), ("CA", "C"), ("C", "O")]
+    if cterm:
+        bonds.append(("C", "OXT"))
+    if res != "GLY":
+        bonds.append(("CA", "CB"))
+    for atom, par, *_ in _SC[res]:
+        bonds.append((atom, par))
+    bonds += _RING.get(res, [])
+    nb = {}
+    for a, b in bonds:
+        nb.setdefault(a, []).append((b, p[b]))
+        nb.setdefault(b, []).append((a, p[a]))
+    if prevp is not None:
+        nb["N"].append(("C-", prevp["C"]))
+    if nextp is not None:
+        nb["C"].append(("N+", nextp["N"]))
+    return nb
+
+
+def _place_h(res, par, hs, p, heavy_nb, nterm):
+    P = p[par]
+    d = _XH[par[0]]
+    nbs = heavy_nb.get(par, [])
+    sp2 = (par == "N" and not nterm) or par in _SP2.get(res, set())
+    units = [(q - P) / np.linalg.norm(q - P) for _, q in nbs]
+    k = len(hs)
+    if len(nbs) >= 2 and k == 1:
+        v = -sum(units)
+        p[hs[0]] = P + d * v / np.linalg.norm(v)
+        return
+    if len(nbs) == 2 and k == 2:
+        bis = -(units[0] + units[1])
+        bis /= np.linalg.norm(bis)
+        perp = np.cross(units[0], units[1])
+        perp /= np.linalg.norm(perp)
+        t = math.radians(54.75)
+        p[hs[0]] = P + d * (math.cos(t) * bis + math.sin(t) * perp)
+        p[hs[1]] = P + d * (math.cos(t) * bis - math.sin(t) * perp)
+        return
+    if len(nbs) == 1:
+        g1name, G1 = nbs[0]
+        # reference atom: any other heavy neighbour of G1
+        G2 = None
+        for nm2, q in heavy_nb.get(g1name, []):
+            if nm2 != par:
+                G2 = q
+                break
+        if G2 is None:
+            G2 = G1 + np.array([1.0, 0.3, 0.1])
+        if sp2:
+            for h, tor in zip(hs, (180.0, 0.0)):
+                p[h] = place(G2, G1, P, d, 120.0, tor)
+        elif k == 3:
+            for h, tor in zip(hs, (60.0, 180.0, 300.0)):
+                p[h] = place(G2, G1, P, d, 109.5, tor)
+        else:
+            angle = 96.0 if par[0] == "S" else 109.5
+            for h, tor in zip(hs, (180.0, 60.0, 300.0)):
+                p[h] = place(G2, G1, P, d, angle, tor)
+        return
+    raise ValueError(f"no hydrogen rule for {res}:{par} ({len(nbs)} heavy, {k} H)")
+
+
+# ───────────────────────────── compaction ───────────────────────────────────
+
+
+def rg_target(n_res: int) -> float:
+    return 2.2 * n_res ** 0.38
+
+
+# Side-chain centroid pseudo-atom of the reduced model: distance from CA
+# along CA->CB and contact radius (A).  Reserves room for the side chain
+# while the backbone is compacted.
+_SC_CENTROID = {
+    "GLY": (0.0, 1.4), "ALA": (1.53, 1.4), "SER": (1.9, 1.5), "CYS": (2.1, 1.6),
+    "THR": (1.9, 1.7), "VAL": (1.95, 1.8), "PRO": (1.9, 1.6), "ASP": (2.4, 1.8),
+    "ASN": (2.4, 1.8), "ILE": (2.3, 2.0), "LEU": (2.6, 2.0), "MET": (2.9, 2.0),
+    "GLU": (2.9, 1.9), "GLN": (2.9, 1.9), "HIS": (3.1, 2.0), "PHE": (3.3, 2.1),
+    "LYS": (3.3, 1.9), "TYR": (3.6, 2.2), "ARG": (3.8, 2.1), "TRP": (3.7, 2.3),
+}
+_BB_RHO = 1.4     # backbone / CB contact radius of the reduced model
+_SC_SCALE = 0.9     # centroid radii scale
+_NRED = 6         # N, CA, C, O, CB, SC per residue
+_GROW_R = 2.0       # growth sphere radius in units of the Rg target
+
+
+def _reduced_finish(seq, X, r, psi):
+    Nn, CA, C = X[r, 0], X[r, 1], X[r, 2]
+    X[r, 3] = place(Nn, CA, C, 1.231, 120.5, psi + 180.0)
+    if seq[r] == "GLY":
+        X[r, 4] = CA
+        X[r, 5] = CA
+    else:
+        X[r, 4] = place(C, Nn, CA, 1.530, 110.5, -122.7)
+        u = X[r, 4] - CA
+        X[r, 5] = CA + _SC_CENTROID[seq[r]][0] * u / np.linalg.norm(u)
+
+
+def _rho(seq):
+    rho = np.full((len(seq), _NRED), _BB_RHO)
+    for r, res in enumerate(seq):
+        rho[r, 5] = _SC_CENTROID[res][1] * _SC_SCALE
+    return rho
+
+
+def _backbone_model(seq, phis, psis):
+    """Reduced model: N, CA, C, O, CB, side-chain centroid per residue
+    (GLY: CB and centroid at CA)."""
+    n = len(seq)
+    X = np.zeros((n, _NRED, 3))
+    ang = math.radians(111.2)
+    for r in range(n):
+        if r == 0:
+            X[0, 0] = (0.0, 0.0, 0.0)
+            X[0, 1] = (1.458, 0.0, 0.0)
+            X[0, 2] = X[0, 1] + 1.525 * np.array([-math.cos(ang), math.sin(ang), 0.0])
+        else:
+            X[r, 0] = place(X[r - 1, 0], X[r - 1, 1], X[r - 1, 2], 1.329, 116.2, psis[r - 1])
+            X[r, 1] = place(X[r - 1, 1], X[r - 1, 2], X[r, 0], 1.458, 121.7, 180.0)
+            X[r, 2] = place(X[r - 1, 2], X[r, 0], X[r, 1], 1.525, 111.2, phis[r])
+        _reduced_finish(seq, X, r, psis[r])
+    return X
+
+
+def _fix_pro(seq, phis, r):
+    """Proline's ring pins phi near -63 deg; the basins' sheet draws would
+    leave the ring open."""
+    if seq[r] == "PRO":
+        phis[r] = -63.0
+
+
+def _grow(rng, seq, phis, psis, tries=20, max_work=None, radius=None):
+    """Grow the chain residue by residue, redrawing phi/psi so that no
+    reduced-model atom of a new residue comes closer than the sum of contact
+    radii to residues more than 2 back and (if ``radius`` is given) its CA
+    stays inside the sphere of that radius about the first CA; when a residue
+    cannot be placed the last few residues are redrawn (backtracking)."""
+    n = len(seq)
+    X = np.zeros((n, _NRED, 3))
+    rho = _rho(seq)
+    ang = math.radians(111.2)
+    X[0, 0] = (0.0, 0.0, 0.0)
+    X[0, 1] = (1.458, 0.0, 0.0)
+    X[0, 2] = X[0, 1] + 1.525 * np.array([-math.cos(ang), math.sin(ang), 0.0])
+
+    def put(r):
+        X[r, 0] = place(X[r - 1, 0], X[r - 1, 1], X[r - 1, 2], 1.329, 116.2, psis[r - 1])
+        X[r, 1] = place(X[r - 1, 1], X[r - 1, 2], X[r, 0], 1.458, 121.7, 180.0)
+        X[r, 2] = place(X[r - 1, 2], X[r, 0], X[r, 1], 1.525, 111.2, phis[r])
+        _reduced_finish(seq, X, r, psis[r])
+
+    def ok(r):
+        if radius is not None and ((X[r, 1] - X[0, 1]) ** 2).sum() > radius * radius:
+            return False
+        if r < 3:
+            return True
+        old = X[:r - 2].reshape(-1, 3)
+        ro = rho[:r - 2].reshape(-1)
+        new = np.concatenate([X[r], X[r - 1, 2:4]])
+        rn = np.concatenate([rho[r], rho[r - 1, 2:4]])
+        d2 = ((old[:, None, :] - new[None, :, :]) ** 2).sum(-1)
+        lim = ro[:, None] + rn[None, :]
+        return bool((d2 >= lim * lim).all())
+
+    _reduced_finish(seq, X, 0, psis[0])
+    work = 0
+    max_work = max_work or 200 * n
+    r = 1
+    fails = [0] * n
+    while r < n and work < max_work:
+        placed = False
+        for t in range(tries):
+            work += 1
+            if t:
+                phis[r], psis[r] = _sample_phipsi(rng)
+                psis[r - 1] = _sample_phipsi(rng)[1]
+                _fix_pro(seq, phis, r)
+                _reduced_finish(seq, X, r - 1, psis[r - 1])
+            put(r)
+            if ok(r):
+                placed = True
+                break
+        if placed:
+            r += 1
+            continue
+        fails[r] += 1
+        back = min(r - 1, 2 + fails[r])
+        r -= back
+        for q in range(r, r + back):
+            phis[q], psis[q] = _sample_phipsi(rng)
+            _fix_pro(seq, phis, q)
+        _reduced_finish(seq, X, r - 1, psis[r - 1])
+    while r < n:        # work cap reached: place the rest as drawn
+        put(r)
+        r += 1
+    return phis, psis
+
+
+def _compact(rng, seq, phis, psis, target, max_props, T=0.02):
+    """Pivot compaction of the reduced model in torsion space.
+
+    A proposal redraws phi or psi of one residue from the basins, rotates the
+    smaller half of the chain about the bond, and is kept only if Rg(CA)
+    does not grow beyond a Metropolis allowance exp(-dRg/T) and no reduced
+    atom of the moved half comes closer than the sum of contact radii to the
+    fixed half (pairs within 2 residues excepted).  Returns (phis, psis, Rg)."""
+    from scipy.spatial import cKDTree
+    n = len(seq)
+    X = _backbone_model(seq, phis, psis)
+    K = _NRED
+    flat = X.reshape(-1, 3)
+    rho = _rho(seq).reshape(-1)
+    res_of = np.repeat(np.arange(n), K)
+    reach = 2.0 * rho.max()
+    ca = flat[1::K].copy()
+
+    def rg_of(c):
+        return math.sqrt(((c - c.mean(0)) ** 2).sum(1).mean())
+
+    rg = rg_of(ca)
+    phis = list(phis)
+    psis = list(psis)
+    props = 0
+    while rg > target and props < max_props:
+        props += 1
+        r = int(rng.integers(1, n - 1))
+        use_phi = rng.random() < 0.5
+        if use_phi and seq[r] == "PRO":
+            continue
+        nphi, npsi = _sample_phipsi(rng)
+        side = res_of > r
+        if use_phi:
+            a, b = flat[K * r], flat[K * r + 1]
+            theta = math.radians(nphi - phis[r])
+            side[K * r + 1:K * r + K] = True
+        else:
+            a, b = flat[K * r + 1], flat[K * r + 2]
+            theta = math.radians(npsi - psis[r])
+            side[K * r + 2] = True
+            side[K * r + 3] = True
+        R = _rotation(b - a, theta)
+        if side.sum() > len(side) // 2:
+            side = ~side
+            R = R.T
+        new_block = (flat[side] - b) @ R.T + b
+        ca_new = ca.copy()
+        ca_side = side[1::K]
+        ca_new[ca_side] = (ca[ca_side] - b) @ R.T + b
+        rg_new = rg_of(ca_new)
+        if rg_new > rg and rng.random() >= math.exp(-(rg_new - rg) / T):
+            continue
+        fx_res = res_of[~side]
+        fx_rho = rho[~side]
+        mv_res = res_of[side]
+        mv_rho = rho[side]
+        d, k = cKDTree(flat[~side]).query(new_block, k=24, distance_upper_bound=reach)
+        hit = np.isfinite(d)
+        if hit.any():
+            kk = np.where(hit, k, 0)
+            lim = fx_rho[kk] + mv_rho[:, None]
+            bad = hit & (d < lim) & (np.abs(fx_res[kk] - mv_res[:, None]) > 2)
+            if bad.any():
+                continue
+        flat[side] = new_block
+        ca = ca_new
+        rg = rg_new
+        if use_phi:
+            phis[r] = nphi
+        else:
+            psis[r] = npsi
+    return phis, psis, rg
+
+
+def _clashing_residues(xyz, names, res_of, bonds):
+    """Residues owning a non-bonded (not 1-2 / 1-3) contact that is too close:
+    H-H < 1.6 A, H-X < 1.9 A, X-X < 2.6 A."""
+    from scipy.spatial import cKDTree
+    N = len(names)
+    adj = [[] for _ in range(N)]
+    for i, j in bonds:
+        adj[i].append(j)
+        adj[j].append(i)
+    near = set()
+    for i in range(N):
+        for j in adj[i]:
+            near.add((min(i, j), max(i, j)))
+            for k in adj[j]:
+                if k != i:
+                    near.add((min(i, k), max(i, k)))
+    isH = np.array([nm[0] == "H" for nm in names])
+    out = set()
+    for i, j in cKDTree(xyz).query_pairs(2.6):
+        if (i, j) in near:
+            continue
+        d = float(np.linalg.norm(xyz[i] - xyz[j]))
+        lim = 1.6 if (isH[i] and isH[j]) else (1.9 if (isH[i] or isH[j]) else 2.6)
+        if d < lim:
+            out.add(int(res_of[i]))
+            out.add(int(res_of[j]))
+    return out
+
+
+_PRO_CHI4 = -10.0   # torsion CB-CA-N-CD (ring closure)
+
+
+def _proline_ring(p, phi):
+    """CD bonded to N, nearly eclipsing CB about CA-N (ring closure), CG
+    bonded to CB and CD, puckered 25 deg out of the ring plane away from CA."""
+    p["CD"] = place(p["CB"], p["CA"], p["N"], 1.474, 111.7, _PRO_CHI4)
+    cb, cd = p["CB"], p["CD"]
+    m = 0.5 * (cb + cd)
+    e = cd - cb
+    h = 0.5 * np.linalg.norm(e)
+    e /= np.linalg.norm(e)
+    w = m - p["CA"]
+    w -= np.dot(w, e) * e
+    w /= np.linalg.norm(w)
+    nrm = np.cross(e, w)
+    rad = math.sqrt(max(1.50 ** 2 - h * h, 0.25))
+    t = math.radians(25.0)
+    p["CG"] = m + rad * (math.cos(t) * w + math.sin(t) * nrm)
+
+
+def _sc_coords(res, p, chi_r, prevp, nextp, nterm, cterm):
+    """Side-chain heavy atoms beyond CB plus every side-chain hydrogen
+    (hydrogens of CB and beyond) for one residue, given backbone+CB in p."""
+    q = {k: p[k] for k in ("N", "CA", "C", "O", "CB") if k in p}
+    if "OXT" in p:
+        q["OXT"] = p["OXT"]
+    for atom, par, a2, a3, bond, angle, tors in _SC[res]:
+        if isinstance(tors, tuple):
+            tors = chi_r[tors[0]] + tors[1]
+        q[atom] = place(q[a3], q[a2], q[par], bond, angle, tors)
+    names = _residue_atoms(res, nterm, cterm)
+    heavy_nb = _heavy_neighbours(res, q, nterm, cterm, prevp, nextp)
+    parent = None
+    hs_of = {}
+    for nm in names:
+        if nm[0] != "H":
+            parent = nm
+        else:
+            hs_of.setdefault(parent, []).append(nm)
+    out = {}
+    for par, hs in hs_of.items():
+        if par in ("N", "CA"):
+            continue
+        _place_h(res, par, hs, q, heavy_nb, nterm=nterm)
+    for nm in names:
+        if nm in q and nm not in ("N", "CA", "C", "O", "OXT", "CB", "H", "HA", "HA2", "HA3",
+                                  "H1", "H2", "H3"):
+            out[nm] = q[nm]
+    return out
+
+
+def _rotamer_grid(rng, res):
+    kinds = _CHI_KIND.get(res, [])
+    grids = [[]]
+    for kind in kinds:
+        grids = [g + [w] for g in grids for w in _WELLS[kind]]
+    cands = []
+    for g in grids:
+        for jit in (0.0, 1.0):
+            cands.append({f"chi{k + 1}": v + (rng.normal(0.0, 12.0) if jit else 0.0)
+                          for k, v in enumerate(g)})
+    return cands
+
+
+def _pack_rotamers(rng, seq, phis, psis, chis, xyz, names, res_of, passes=4):
+    """Greedy rotamer search: for every residue owning a close contact,
+    try all chi wells (with and without jitter) and keep the rotamer with the
+    least overlap score sum(max(0, 0.8*d0 - r)^2) against all other atoms,
+    d0 = 3.2 A heavy-heavy, 2.6 A with one H, 2.2 A H-H."""
+    from scipy.spatial import cKDTree
+    n = len(seq)
+    N = len(names)
+    isH = np.array([nm[0] == "H" for nm in names])
+    idx_of = {}
+    for i, (r, nm) in enumerate(zip(res_of, names)):
+        idx_of[(int(r), nm)] = i
+    # backbone dict per residue for _sc_coords
+    def bb(r):
+        d = {}
+        for nm in ("N", "CA", "C", "O", "OXT", "CB"):
+            k = idx_of.get((r, nm))
+            if k is not None:
+                d[nm] = xyz[k]
+        return d
+
+    def score(tree, pts, hflag, own):
+        tot = 0.0
+        nbrs = tree.query_ball_point(pts, 3.2 * 0.8)
+        for a, lst in enumerate(nbrs):
+            for b in lst:
+                if own[b]:
+                    continue
+                d0 = 3.2 if not (hflag[a] or isH[b]) else (2.2 if (hflag[a] and isH[b]) else 2.6)
+                r = float(np.linalg.norm(pts[a] - xyz[b]))
+                if r < 0.8 * d0:
+                    tot += (0.8 * d0 - r) ** 2
+        return tot
+
+    for _ in range(passes):
+        tree = cKDTree(xyz)
+        changed = False
+        for r in range(n):
+            res = seq[r]
+            if not _CHI_KIND.get(res):
+                continue
+            own = res_of == r
+            for nm_bb in ("O", "C", "H", "OXT"):
+                k_bb = idx_of.get((r, nm_bb))
+                if k_bb is not None:
+                    own = own.copy()
+                    own[k_bb] = False
+            cur_pts = _sc_coords(res, bb(r), chis[r], bb(r - 1) if r else None,
+                                 bb(r + 1) if r + 1 < n else None, r == 0, r == n - 1)
+            keys = list(cur_pts)
+            pts = np.array([cur_pts[k] for k in keys])
+            hflag = np.array([k[0] == "H" for k in keys])
+            best = score(tree, pts, hflag, own)
+            if best == 0.0:
+                continue
+            best_chi = None
+            for cand in _rotamer_grid(rng, res):
+                c = _sc_coords(res, bb(r), cand, bb(r - 1) if r else None,
+                               bb(r + 1) if r + 1 < n else None, r == 0, r == n - 1)
+                sc = score(tree, np.array([c[k] for k in keys]), hflag, own)
+                if sc < best - 1e-9:
+                    best, best_chi, best_pts = sc, cand, c
+            if best_chi is not None:
+                chis[r] = best_chi
+                for k in keys:
+                    xyz[idx_of[(r, k)]] = best_pts[k]
+                changed = True
+        if not changed:
+            break
+    return xyz
+
+
+# ───────────────────────────── public builders ──────────────────────────────
+
+
+def _params(resnames, atomnames):
+    q = np.empty(len(atomnames))
+    r = np.empty(len(atomnames))
+    e = np.empty(len(atomnames))
+    for k, (rn, an) in enumerate(zip(resnames, atomnames)):
+        q[k], r[k], e[k] = atom_params(rn, an)
+    return q, r, e
+
+
+def make_chain(n_res: int, seed: int, compact: bool = True, res_offset: int = 0,
+               max_props: int | None = None, seq=None):
+    """One seeded chain.  Returns a system dict (see module docstring)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if seq is None:
+        seq = [STANDARD_20[int(k)] for k in rng.integers(0, 20, size=n_res)]
+    phis, psis = (list(t) for t in zip(*[_sample_phipsi(rng) for _ in range(n_res)]))
+    for r in range(n_res):
+        _fix_pro(seq, phis, r)
+    chis = [_sample_chis(rng, s) for s in seq]
+    rg = None
+    if compact and n_res > 3:
+        target = rg_target(n_res)
+        if max_props is None:
+            max_props = 200 * n_res + 20000
+        phis, psis = _grow(rng, seq, phis, psis, radius=_GROW_R * target,
+                           max_work=2000 * n_res)
+        phis, psis, rg = _compact(rng, seq, phis, psis, target, max_props)
+    xyz, names, res_of, bonds = _build_chain(rng, seq, phis, psis, chis)
+    xyz = _pack_rotamers(rng, seq, phis, psis, chis, xyz, names, res_of)
+    resnames = [seq[r] for r in res_of]
+    res_idx = [res_offset + int(r) for r in res_of]
+    xyz = xyz - xyz.mean(0)
+    q, rs, e = _params(resnames, names)
+    return {"xyz": xyz, "charge": q, "rstar": rs, "eps": e,
+            "resnames": resnames, "atomnames": names,
+            "res_idx": np.array(res_idx, dtype=np.int32),
+            "ss_pairs": np.zeros((0, 2), dtype=np.int32),
+            "seq": list(seq), "rg_ca": rg}
+
+
+def concat(systems, gap: int = 2):
+    """Concatenate chains; res_idx of each chain starts >= gap past the last
+    one so no false peptide bond joins chains (src/physics_engine.cpp:1434)."""
+    out = {k: [] for k in ("xyz", "charge", "rstar", "eps", "resnames",
+                           "atomnames", "res_idx", "ss_pairs")}
+    base_atom = 0
+    base_res = 0
+    for s in systems:
+        ri = s["res_idx"] - s["res_idx"].min() + base_res
+        out["res_idx"].append(ri)
+        base_res = int(ri.max()) + 1 + gap
+        for k in ("xyz", "charge", "rstar", "eps"):
+            out[k].append(s[k])
+        out["resnames"] += s["resnames"]
+        out["atomnames"] += s["atomnames"]
+        out["ss_pairs"].append(s["ss_pairs"] + base_atom)
+        base_atom += len(s["charge"])
+    res = {k: np.concatenate(out[k]) for k in ("xyz", "charge", "rstar", "eps", "res_idx")}
+    res["res_idx"] = res["res_idx"].astype(np.int32)
+    res["resnames"] = out["resnames"]
+    res["atomnames"] = out["atomnames"]
+    res["ss_pairs"] = np.concatenate(out["ss_pairs"]).astype(np.int32).reshape(-1, 2)
+    return res
+
+
+def make_complex(n_chains: int = 4, n_res: int = 500, seed: int = 5,
+                 interface: float = 8.0, n_ss: int = 12):
+    """Multi-chain complex: chains packed on a tetrahedron so neighbouring
+    chain surfaces (Rg-sphere radius sqrt(5/3)*Rg) sit ``interface`` A apart,
+    then ``n_ss`` SG-SG pairs (nearest CYS pairs across the complex) moved to
+    2.04 A by translating the second cysteine's SG along the SG-SG line.
+    The moved cysteines lose HG (a disulfide has no thiol H)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    chains = [make_chain(n_res, int(rng.integers(1 << 30))) for _ in range(n_chains)]
+    rad = max(math.sqrt(5.0 / 3.0) * rg_target(n_res), 1.0)
+    sep = 2 * rad + interface
+    tet = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float)
+    tet *= sep / np.linalg.norm(tet[0] - tet[1])
+    for c, s in enumerate(chains):
+        s["xyz"] = s["xyz"] + tet[c % 4] * (1 + c // 4)
+    sysd = concat(chains)
+    if n_ss:
+        sysd = _add_disulfides(sysd, n_ss)
+    return sysd
+
+
+def _add_disulfides(sysd, n_ss):
+    names = sysd["atomnames"]
+    rn = sysd["resnames"]
+    sg = [i for i, (a, r) in enumerate(zip(names, rn)) if a == "SG" and r == "CYS"]
+    xyz = sysd["xyz"]
+    cand = []
+    for a in range(len(sg)):
+        for b in range(a + 1, len(sg)):
+            i, j = sg[a], sg[b]
+            if sysd["res_idx"][i] == sysd["res_idx"][j]:
+                continue
+            cand.append((float(np.linalg.norm(xyz[i] - xyz[j])), i, j))
+    cand.sort()
+    used = set()
+    pairs = []
+    for _, i, j in cand:
+        if i in used or j in used:
+            continue
+        used.add(i)
+        used.add(j)
+        pairs.append((i, j))
+        if len(pairs) == n_ss:
+            break
+    drop = set()
+    for i, j in pairs:
+        v = xyz[j] - xyz[i]
+        xyz[j] = xyz[i] + 2.04 * v / np.linalg.norm(v)
+        for k in (i, j):  # thiol H of SG is the atom right after SG
+            if k + 1 < len(names) and names[k + 1] == "HG":
+                drop.add(k + 1)
+    keep = np.array([k for k in range(len(names)) if k not in drop])
+    remap = -np.ones(len(names), dtype=np.int64)
+    remap[keep] = np.arange(len(keep))
+    out = {k: sysd[k][keep] for k in ("xyz", "charge", "rstar", "eps", "res_idx")}
+    out["resnames"] = [rn[k] for k in keep]
+    out["atomnames"] = [names[k] for k in keep]
+    out["ss_pairs"] = np.array([(remap[i], remap[j]) for i, j in pairs],
+                               dtype=np.int32).reshape(-1, 2)
+    return out
+
+
+def make_pt_set(n_proteins: int = 64, seed: int = 4, median: int = 180,
+                sigma: float = 0.6, lo: int = 40, hi: int = 800, scale: float = 1.0):
+    """Residue counts ~ LogNormal(ln median, sigma) truncated to [lo, hi]
+    (redrawn outside), each compacted; ``scale`` shrinks sizes for tests."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(n_proteins):
+        while True:
+            n = int(round(rng.lognormal(math.log(median), sigma)))
+            if lo <= n <= hi:
+                break
+        n = max(4, int(round(n * scale)))
+        out.append(make_chain(n, int(rng.integers(1 << 30))))
+    return out
+
+
+def temperature_ladder(n: int = 8, t_lo: float = 280.0, t_hi: float = 450.0):
+    """Geometric ladder in kelvin."""
+    return [t_lo * (t_hi / t_lo) ** (k / (n - 1)) for k in range(n)]
+
+
+KB_KCAL = 0.0019872041   # kcal/(mol K)
+
+
+def kelvin_to_kcal(t_kelvin: float) -> float:
+    """Reference API temperatures are kT in kcal/mol (physics_engine.cpp:68)."""
+    return KB_KCAL * t_kelvin
