@@ -1,0 +1,132 @@
+/*
+ * alma.h -- C ABI of the MI355X (gfx950) energy-and-sampling core behind
+ * ALMA's `protein_physics` module.
+ *
+ * Plain pointers and sizes only; no C++ exceptions or torch types cross this
+ * boundary.  Every call returns an alma_status (0 = OK); the message of the
+ * last failure is available from alma_last_error().  A context owns all
+ * device memory of one system and is single-threaded, mirroring the
+ * reference's one-PhysicsEngine-per-thread rule (python/gui_main.py:1691-1696).
+ * Host buffers are copied synchronously.
+ */
+#ifndef ALMA_H
+#define ALMA_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int alma_status;
+enum {
+    ALMA_OK = 0,
+    ALMA_EINVAL = 1,   /* bad argument           -> Python ValueError   */
+    ALMA_ERUNTIME = 2, /* HIP / capacity failure -> Python RuntimeError */
+    ALMA_ENODEV = 3    /* no usable GPU          -> Python RuntimeError */
+};
+
+typedef struct alma_ctx alma_ctx;
+
+/* Energy terms reported by alma_energy (kcal/mol). */
+enum { ALMA_DH = 0, ALMA_GB, ALMA_LJ, ALMA_HC, ALMA_SASA, ALMA_DIH, ALMA_SS, ALMA_TOTAL,
+       ALMA_NTERMS };
+
+/* Topology in flat CSR form.  Field meanings are those of BondTopology's
+ * exported arrays (src/physics_engine.cpp:3729-3841): exclusions excl[i]
+ * (sorted, j > i), rotatable bonds (i, j, kind, lever-arm scale), j-side
+ * atom sets (DFS order), dihedral records with Fourier terms, disulfide
+ * pairs, crankshaft (phi, psi) bond pairs and concerted side-chain bond
+ * pairs.  Counts may be zero (pointers then unused). */
+typedef struct {
+    int32_t n_atoms;
+    const int32_t* excl_off;  const int32_t* excl_idx;
+    int32_t n_rot;
+    const int32_t* rb_i; const int32_t* rb_j; const int32_t* rb_kind; const double* rb_scale;
+    const int32_t* side_off; const int32_t* side_idx;
+    int32_t n_dih;
+    const int32_t* dih_a; const int32_t* dih_b; const int32_t* dih_c; const int32_t* dih_d;
+    const int32_t* dih_toff; const double* dih_v2; const int32_t* dih_n; const double* dih_gamma;
+    int32_t n_ss;  const int32_t* ss_i;  const int32_t* ss_j;
+    int32_t n_cp;  const int32_t* cp_phi; const int32_t* cp_psi;
+    int32_t n_csp; const int32_t* csp_a; const int32_t* csp_b;
+} alma_topo;
+
+/* Context lifetime.  Replaces PhysicsEngine::PhysicsEngine()
+ * (src/physics_engine.cpp:2236; GPU ctor src/physics_engine_cuda.cu:1094-1101,
+ * which throws when no device is present: here ALMA_ENODEV). */
+alma_status alma_create(int device, alma_ctx** out);
+void        alma_destroy(alma_ctx* ctx);
+const char* alma_last_error(const alma_ctx* ctx);
+/* PhysicsEngine.device_name (src/physics_engine_cuda.cu:1394-1395). */
+alma_status alma_device_name(alma_ctx* ctx, char* buf, int32_t len);
+
+/* Load particles (x,y,z,charge,radius=R*,epsilon; Particle,
+ * src/physics_engine.cpp:208-214) and an optional topology.  topo == NULL
+ * is calculate_potential(particles) without topology: no exclusions,
+ * dihedrals or disulfides (src/physics_engine.cpp:2242-2248). */
+alma_status alma_set_system(alma_ctx* ctx, int32_t n, const double* xyz, const double* q,
+                            const double* rstar, const double* eps, const alma_topo* topo);
+/* Replace coordinates only (3n doubles). */
+alma_status alma_set_positions(alma_ctx* ctx, const double* xyz);
+
+/* Full evaluation = PhysicsEngine::calculate_potential
+ * (src/physics_engine.cpp:2242-2248): fresh neighbour list, full HCT Born
+ * radii, total_e (:2205-2234).  terms[ALMA_NTERMS]; TOTAL is the drop-in
+ * value. */
+alma_status alma_energy(alma_ctx* ctx, double* terms);
+
+/* Neighbour list of the last evaluation in the reference's cell-scan order
+ * (NeighborList::build, src/physics_engine.cpp:293-342).  offs[n+1]; js is
+ * written only when js_cap >= *npairs. */
+alma_status alma_nlist(alma_ctx* ctx, int32_t* offs, int32_t* js, int64_t js_cap,
+                       int64_t* npairs);
+/* Born radii of the last evaluation (born_radii, src/physics_engine.cpp:2024-2039). */
+alma_status alma_born_radii(alma_ctx* ctx, double* a);
+
+/* ---- device-resident Metropolis MC ---- */
+enum {
+    ALMA_MC_ENSEMBLE = 0,   /* generate_ensemble chains (src/physics_engine.cpp:2273-2543) */
+    ALMA_MC_LANDSCAPE = 1,  /* run_landscape_trajectory / _segment moves (:2556-3270)      */
+    ALMA_MC_TORSION30 = 2,  /* one uniformly chosen torsion, delta ~ U[-pi/6, pi/6]         */
+    ALMA_MC_MALA = 3        /* gradient-guided torsion proposals (no reference counterpart) */
+};
+
+typedef struct {
+    int32_t mode;
+    int32_t n_chains;        /* independent chains; chain c draws from mt19937(seed + c) */
+    int32_t steps;           /* MC steps per chain                                      */
+    int32_t snapshot_every;  /* landscape snapshots (0 = none)                          */
+    double temperature;      /* kT, kcal/mol (reference convention)                     */
+    double max_angle;        /* initial adaptive step cur_max (rad)                     */
+    double mala_step;        /* MALA torsion step (rad)                                 */
+    uint32_t seed;
+    int32_t trace;           /* record per-step trace (chain 0) when nonzero            */
+} alma_mc_params;
+
+typedef struct {
+    double cur_e;            /* running energy                                           */
+    double cur_max;          /* final adaptive step                                      */
+    int64_t accepted;
+    int64_t moves;
+    int64_t rebuilds;
+    int64_t words;           /* mt19937 words consumed                                   */
+} alma_mc_stats;
+
+/* Run n_chains chains from the loaded state; xyz_out: n_chains*3n doubles,
+ * stats: n_chains entries.  snap_xyz (n_chains*n_snap*3n) / snap_e
+ * (n_chains*n_snap) receive landscape snapshots with fresh energies. */
+alma_status alma_mc_run(alma_ctx* ctx, const alma_mc_params* p, double* xyz_out,
+                        alma_mc_stats* stats, double* snap_xyz, double* snap_e);
+/* Trace of chain 0 of the last alma_mc_run with trace != 0 (length steps):
+ * move kind, bond / pair index, delta, dE, accepted. */
+alma_status alma_mc_trace(alma_ctx* ctx, int32_t* kind, int32_t* bond, double* delta,
+                          double* dE, int32_t* accepted);
+
+/* ---- forces (no reference counterpart) ---- */
+/* E, dE/dx (3n) with the Born-radius chain rule, dE/dtheta per rotatable
+ * bond (torque about each bond for its j-side), at the loaded positions. */
+alma_status alma_forces(alma_ctx* ctx, double* energy, double* dEdx, double* dEdtheta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,220 @@
+"""ctypes binding of libalma.so (include/alma.h).
+
+Used by the torch autograd wrapper, the batched parallel-tempering driver and
+the tests.  Loading fails loudly when the library is not built: there is no
+CPU fallback anywhere in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libalma.so")
+
+ALMA_OK, ALMA_EINVAL, ALMA_ERUNTIME, ALMA_ENODEV = 0, 1, 2, 3
+TERMS = ("DH", "GB", "LJ", "HC", "SASA", "DIH", "SS", "TOTAL")
+MC_ENSEMBLE, MC_LANDSCAPE, MC_TORSION30, MC_MALA = 0, 1, 2, 3
+
+_I = C.POINTER(C.c_int32)
+_D = C.POINTER(C.c_double)
+
+
+class Topo(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("excl_off", _I), ("excl_idx", _I),
+                ("n_rot", C.c_int32), ("rb_i", _I), ("rb_j", _I), ("rb_kind", _I),
+                ("rb_scale", _D), ("side_off", _I), ("side_idx", _I),
+                ("n_dih", C.c_int32), ("dih_a", _I), ("dih_b", _I), ("dih_c", _I),
+                ("dih_d", _I), ("dih_toff", _I), ("dih_v2", _D), ("dih_n", _I),
+                ("dih_gamma", _D), ("n_ss", C.c_int32), ("ss_i", _I), ("ss_j", _I),
+                ("n_cp", C.c_int32), ("cp_phi", _I), ("cp_psi", _I),
+                ("n_csp", C.c_int32), ("csp_a", _I), ("csp_b", _I)]
+
+
+class McParams(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("n_chains", C.c_int32), ("steps", C.c_int32),
+                ("snapshot_every", C.c_int32), ("temperature", C.c_double),
+                ("max_angle", C.c_double), ("mala_step", C.c_double), ("seed", C.c_uint32),
+                ("trace", C.c_int32)]
+
+
+class McStats(C.Structure):
+    _fields_ = [("cur_e", C.c_double), ("cur_max", C.c_double), ("accepted", C.c_int64),
+                ("moves", C.c_int64), ("rebuilds", C.c_int64), ("words", C.c_int64)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libalma.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.alma_create.argtypes = [C.c_int, C.POINTER(P)]
+        L.alma_destroy.argtypes = [P]
+        L.alma_destroy.restype = None
+        L.alma_last_error.argtypes = [P]
+        L.alma_last_error.restype = C.c_char_p
+        L.alma_device_name.argtypes = [P, C.c_char_p, C.c_int32]
+        L.alma_set_system.argtypes = [P, C.c_int32, _D, _D, _D, _D, C.POINTER(Topo)]
+        L.alma_set_positions.argtypes = [P, _D]
+        L.alma_energy.argtypes = [P, _D]
+        L.alma_nlist.argtypes = [P, _I, _I, C.c_int64, C.POINTER(C.c_int64)]
+        L.alma_born_radii.argtypes = [P, _D]
+        L.alma_mc_run.argtypes = [P, C.POINTER(McParams), _D, C.POINTER(McStats), _D, _D]
+        L.alma_mc_trace.argtypes = [P, _I, _I, _D, _D, _I]
+        L.alma_forces.argtypes = [P, _D, _D, _D]
+        _LIB = L
+    return _LIB
+
+
+EXPORTED = ("alma_create", "alma_destroy", "alma_last_error", "alma_device_name",
+            "alma_set_system", "alma_set_positions", "alma_energy", "alma_nlist",
+            "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_forces")
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class AlmaError(RuntimeError):
+    pass
+
+
+class TopoArrays:
+    """Owns the arrays behind an alma_topo view (dict keys as in the oracle)."""
+
+    INT = ("excl_off", "excl_idx", "rb_i", "rb_j", "rb_kind", "side_off", "side_idx", "dih_a",
+           "dih_b", "dih_c", "dih_d", "dih_toff", "dih_n", "ss_i", "ss_j", "cp_phi", "cp_psi",
+           "csp_a", "csp_b")
+    DBL = ("rb_scale", "dih_v2", "dih_gamma")
+
+    def __init__(self, csr: dict, n_atoms: int):
+        self.keep = {}
+        t = Topo()
+        t.n_atoms = n_atoms
+        for k in self.INT:
+            a = np.ascontiguousarray(csr.get(k, np.zeros(0)), dtype=np.int32)
+            a = a if a.size else np.zeros(1, np.int32)
+            self.keep[k] = a
+            setattr(t, k, _p(a, C.c_int32))
+        for k in self.DBL:
+            a = np.ascontiguousarray(csr.get(k, np.zeros(0)), dtype=np.float64)
+            a = a if a.size else np.zeros(1)
+            self.keep[k] = a
+            setattr(t, k, _p(a, C.c_double))
+        t.n_rot = len(csr.get("rb_i", ()))
+        t.n_dih = len(csr.get("dih_a", ()))
+        t.n_ss = len(csr.get("ss_i", ()))
+        t.n_cp = len(csr.get("cp_phi", ()))
+        t.n_csp = len(csr.get("csp_a", ()))
+        self.struct = t
+
+
+class Context:
+    """One device context (one system).  Single-threaded."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        st = lib().alma_create(device, C.byref(self._h))
+        if st != ALMA_OK:
+            raise AlmaError("alma_create failed (status %d)" % st)
+        self._topo = None
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            lib().alma_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != ALMA_OK:
+            msg = lib().alma_last_error(self._h).decode()
+            if st == ALMA_EINVAL:
+                raise ValueError(msg)
+            raise AlmaError(msg)
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        self._check(lib().alma_device_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    def set_system(self, xyz, q, rstar, eps, topo_csr=None):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        self.n = len(xyz)
+        self._arrs = [xyz, np.ascontiguousarray(q, np.float64),
+                      np.ascontiguousarray(rstar, np.float64), np.ascontiguousarray(eps, np.float64)]
+        self._topo = TopoArrays(topo_csr, self.n) if topo_csr is not None else None
+        self._check(lib().alma_set_system(
+            self._h, self.n, *[_p(a, C.c_double) for a in self._arrs],
+            C.byref(self._topo.struct) if self._topo else None))
+
+    def set_positions(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        self._check(lib().alma_set_positions(self._h, _p(xyz, C.c_double)))
+
+    def energy(self) -> dict:
+        out = np.zeros(8)
+        self._check(lib().alma_energy(self._h, _p(out, C.c_double)))
+        return dict(zip(TERMS, out.tolist()))
+
+    def nlist(self):
+        offs = np.zeros(self.n + 1, np.int32)
+        npairs = C.c_int64()
+        self._check(lib().alma_nlist(self._h, _p(offs, C.c_int32), None, 0, C.byref(npairs)))
+        js = np.zeros(max(npairs.value, 1), np.int32)
+        self._check(lib().alma_nlist(self._h, _p(offs, C.c_int32), _p(js, C.c_int32),
+                                     npairs.value, C.byref(npairs)))
+        return offs, js[:npairs.value]
+
+    def born_radii(self):
+        a = np.zeros(self.n)
+        self._check(lib().alma_born_radii(self._h, _p(a, C.c_double)))
+        return a
+
+    def mc_run(self, mode, steps, n_chains=1, temperature=0.6, max_angle=0.12, seed=5489,
+               snapshot_every=0, mala_step=0.02, trace=False):
+        p = McParams(mode, n_chains, steps, snapshot_every, temperature, max_angle, mala_step,
+                     seed, 1 if trace else 0)
+        xyz = np.zeros((n_chains, self.n, 3))
+        st = (McStats * n_chains)()
+        nsnap = steps // snapshot_every if snapshot_every else 0
+        sx = np.zeros((n_chains, max(nsnap, 1), self.n, 3))
+        se = np.zeros((n_chains, max(nsnap, 1)))
+        self._check(lib().alma_mc_run(self._h, C.byref(p), _p(xyz, C.c_double), st,
+                                      _p(sx, C.c_double) if nsnap else None,
+                                      _p(se, C.c_double) if nsnap else None))
+        stats = [{f: getattr(s, f) for f, _ in McStats._fields_} for s in st]
+        out = {"xyz": xyz, "stats": stats}
+        if nsnap:
+            out["snapshots"] = sx[:, :nsnap]
+            out["energies"] = se[:, :nsnap]
+        if trace:
+            tr = {"kind": np.zeros(steps, np.int32), "bond": np.zeros(steps, np.int32),
+                  "delta": np.zeros(steps), "dE": np.zeros(steps),
+                  "accepted": np.zeros(steps, np.int32)}
+            self._check(lib().alma_mc_trace(self._h, _p(tr["kind"], C.c_int32),
+                                            _p(tr["bond"], C.c_int32), _p(tr["delta"], C.c_double),
+                                            _p(tr["dE"], C.c_double), _p(tr["accepted"], C.c_int32)))
+            out["trace"] = tr
+        return out
+
+    def forces(self):
+        e = C.c_double()
+        g = np.zeros((self.n, 3))
+        nrot = self._topo.struct.n_rot if self._topo else 0
+        gt = np.zeros(max(nrot, 1))
+        self._check(lib().alma_forces(self._h, C.byref(e), _p(g, C.c_double),
+                                      _p(gt, C.c_double) if nrot else None))
+        return e.value, g, gt[:nrot]

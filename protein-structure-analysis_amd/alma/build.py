@@ -1,0 +1,66 @@
+"""Build the native parts in-tree.
+
+  libalma.so            gfx950 HIP kernels + the C ABI (include/alma.h), hipcc
+  protein_physics*.so   the drop-in pybind11 module over the C ABI, g++
+  oracle/liboracle.so   (+ oracle/_ref when the reference sources exist):
+                        test-infrastructure builds, via oracle/Makefile
+
+Every call compiles; nothing is cached.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sysconfig
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))   # protein-structure-analysis_amd
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libalma.so")
+EXT = sysconfig.get_config_var("EXT_SUFFIX")
+MODULE = os.path.join(PKG, "protein_physics" + EXT)
+
+HIP_SOURCES = ["energy.hip", "mc.hip", "forces.hip", "alma_abi.cpp"]
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-munsafe-fp-atomics", "-Wno-unused-result"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed: %s\n%s" % (" ".join(cmd), r.stdout[-6000:]))
+    return r.stdout
+
+
+def build_lib():
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB + ".tmp"
+    _run(["hipcc"] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs)
+    os.replace(tmp, LIB)
+
+
+def build_module():
+    import pybind11
+    inc = sysconfig.get_paths()["include"]
+    tmp = MODULE + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+          "-I", inc, "-I", pybind11.get_include(), "-I", os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "protein_physics.cpp"), os.path.join(CSRC, "topology.cpp"),
+          "-L", PKG, "-lalma", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, MODULE)
+
+
+def build_oracle():
+    _run(["make", "-C", os.path.join(ROOT, "oracle"), "-j4"])
+
+
+def build_all():
+    build_lib()
+    build_module()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB, MODULE)

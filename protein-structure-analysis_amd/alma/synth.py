@@ -195,7 +195,8 @@ def _residue_atoms(res, nterm, cterm):
         i = names.index("N")
         # drop backbone H, add H1 H2 H3 right after N
         names = [a for a in names if a != "H"]
-        names = names[:i + 1] + ["H1", "H2", "H3"] + names[i + 1:]
+        hs = ["H2", "H3"] if res == "PRO" else ["H1", "H2", "H3"]   # N-terminal Pro: NH2+
+        names = names[:i + 1] + hs + names[i + 1:]
     if cterm:
         names.append("OXT")
     return names

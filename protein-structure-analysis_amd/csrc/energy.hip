@@ -1,0 +1,481 @@
+// energy.hip -- gfx950 kernels of the full evaluation path
+// (calculate_potential, src/physics_engine.cpp:2242-2248):
+//   neighbour list (cell binning, reference cell-scan order, FP64 test)
+//   HCT Born radii (FP32 integrand, FP64 per-atom reduction)
+//   fused DH + GB + LJ + hard-core pair sum (FP32 arithmetic, FP64 sums)
+//   SASA half-list fold, dihedral and disulfide terms, final FP64 reduction.
+// One 64-lane wavefront owns one atom's neighbour row; rows are contiguous
+// (n x cap layout) so lanes read consecutive entries.
+#include "alma_internal.h"
+
+#include <cmath>
+
+namespace alma {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;            // 4 wavefronts
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Reference FP64 squared distance, no contraction into FMA (the reference's
+// d^2 < 196 and d^2 > 144 decisions must be reproduced exactly).
+__device__ __forceinline__ double dist2_exact(const double* pi, const double* pj) {
+#pragma clang fp contract(off)
+    double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// ── neighbour list ──────────────────────────────────────────────────────────
+// NeighborList::build, src/physics_engine.cpp:293-342.
+
+__global__ void k_bbox(const double* __restrict__ pos, int n, Grid* g) {
+    __shared__ double slo[3][kBlock], shi[3][kBlock];
+    int t = threadIdx.x;
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = pos[d];
+    for (int i = t; i < n; i += blockDim.x)
+        for (int d = 0; d < 3; ++d) {
+            double v = pos[4 * i + d];
+            lo[d] = fmin(lo[d], v);
+            hi[d] = fmax(hi[d], v);
+        }
+    for (int d = 0; d < 3; ++d) {
+        slo[d][t] = lo[d];
+        shi[d][t] = hi[d];
+    }
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int d = 0; d < 3; ++d) {
+                slo[d][t] = fmin(slo[d][t], slo[d][t + s]);
+                shi[d][t] = fmax(shi[d][t], shi[d][t + s]);
+            }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma clang fp contract(off)
+        const double cs = g->cs;
+        int ncell = 1;
+        for (int d = 0; d < 3; ++d) {
+            double l = slo[d][0] - cs;
+            double len = shi[d][0] - l + 2.0 * cs;
+            int c = (int)ceil(len / cs);
+            g->lo[d] = l;
+            g->nc[d] = c < 1 ? 1 : c;
+            ncell *= g->nc[d];
+        }
+        g->ncell = ncell;
+    }
+}
+
+__global__ void k_cell_assign(const double* __restrict__ pos, int n, const Grid* __restrict__ g,
+                              int max_cells, int* __restrict__ cell_of, int* __restrict__ cnt,
+                              int* __restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        int v = (int)floor((pos[4 * i + d] - g->lo[d]) / g->cs);
+        c[d] = min(g->nc[d] - 1, max(0, v));
+    }
+    int id = c[0] + g->nc[0] * (c[1] + g->nc[1] * c[2]);
+    if (g->ncell > max_cells) {
+        atomicOr(&flags[0], 2);
+        id = 0;
+    }
+    cell_of[i] = id;
+    atomicAdd(&cnt[id], 1);
+}
+
+// exclusive scan of cell counts (single block)
+__global__ void k_cell_scan(const Grid* __restrict__ g, int max_cells, int* __restrict__ cnt,
+                            int* __restrict__ start) {
+    __shared__ int buf[1024];
+    int ncell = min(g->ncell, max_cells);
+    int t = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < ncell; base += 1024) {
+        int v = (base + t < ncell) ? cnt[base + t] : 0;
+        buf[t] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int add = t >= o ? buf[t - o] : 0;
+            __syncthreads();
+            buf[t] += add;
+            __syncthreads();
+        }
+        if (base + t < ncell) start[base + t] = carry + buf[t] - v;
+        int tot = buf[1023];
+        __syncthreads();
+        carry += tot;
+    }
+    if (t == 0) start[ncell] = carry;
+    for (int k = t; k < ncell; k += 1024) cnt[k] = 0;   // reused as fill cursor
+}
+
+__global__ void k_cell_place(int n, const int* __restrict__ cell_of, const int* __restrict__ start,
+                             int* __restrict__ cursor, int* __restrict__ tmp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = cell_of[i];
+    tmp[start[c] + atomicAdd(&cursor[c], 1)] = i;
+}
+
+// ascending atom order inside each cell (the reference push_back order):
+// the rank of atom i is the number of smaller indices in its cell
+__global__ void k_cell_rank(int n, const int* __restrict__ cell_of, const int* __restrict__ start,
+                            const int* __restrict__ tmp, int* __restrict__ sorted) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = cell_of[i];
+    int b = start[c], e = start[c + 1];
+    int r = 0;
+    for (int k = b; k < e; ++k) r += tmp[k] < i;
+    sorted[b + r] = i;
+}
+
+__device__ __forceinline__ bool in_sorted(const int* __restrict__ v, int b, int e, int key) {
+    while (b < e) {
+        int m = (b + e) >> 1;
+        int x = v[m];
+        if (x == key) return true;
+        if (x < key)
+            b = m + 1;
+        else
+            e = m;
+    }
+    return false;
+}
+
+// one wavefront per atom i: scan the 27 cells dz -> dy -> dx, cell members in
+// ascending order, keep j > i with d^2 < 196; ballot + prefix count keeps the
+// scan order when appending.
+__global__ void __launch_bounds__(kBlock) k_nl_fill(const double* __restrict__ pos, int n,
+                                                   const Grid* __restrict__ g,
+                                                   const int* __restrict__ cell_of,
+                                                   const int* __restrict__ start,
+                                                   const int* __restrict__ atoms,
+                                                   const int* __restrict__ excl_off,
+                                                   const int* __restrict__ excl_idx, int cap,
+                                                   uint32_t* __restrict__ nl, int* __restrict__ cnt,
+                                                   int* __restrict__ flags) {
+    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (i >= n || (flags[0] & 2)) return;   // grid larger than its allocation
+    int lane = lane_id();
+    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+    int c = cell_of[i];
+    int cx = c % g->nc[0], cy = (c / g->nc[0]) % g->nc[1], cz = c / (g->nc[0] * g->nc[1]);
+    int eb = 0, ee = 0;
+    if (excl_off) {
+        eb = excl_off[i];
+        ee = excl_off[i + 1];
+    }
+    uint32_t* row = nl + (size_t)i * cap;
+    int count = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        if (z < 0 || z >= g->nc[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            if (y < 0 || y >= g->nc[1]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                int x = cx + dx;
+                if (x < 0 || x >= g->nc[0]) continue;
+                int cc = x + g->nc[0] * (y + g->nc[1] * z);
+                int b = start[cc], e = start[cc + 1];
+                for (int base = b; base < e; base += kWave) {
+                    int k = base + lane;
+                    bool keep = false;
+                    int j = 0;
+                    if (k < e) {
+                        j = atoms[k];
+                        if (j > i) keep = dist2_exact(pi, pos + 4 * j) < kNlRcut2;
+                    }
+                    uint64_t m = __ballot(keep);
+                    if (keep) {
+                        int slot = count + __popcll(m & ((1ull << lane) - 1ull));
+                        uint32_t v = (uint32_t)j;
+                        if (excl_off && in_sorted(excl_idx, eb, ee, j)) v |= kExclBit;
+                        if (slot < cap) row[slot] = v;
+                    }
+                    count += __popcll(m);
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        cnt[i] = count;
+        atomicMax(&flags[1], count);
+        if (count > cap) atomicOr(&flags[0], 1);
+    }
+}
+
+// ── HCT Born radii ──────────────────────────────────────────────────────────
+// hct, src/physics_engine.cpp:2007-2011 (as written, FP32);
+// born_radii :2024-2039 sums both directions over every neighbour entry.
+__device__ __forceinline__ float hct_f(float r, float r2, float ri, float rj) {
+    float L = fmaxf(fabsf(r - rj), ri), U = r + rj;
+    if (ri >= U) return 0.0f;
+    return 1.0f / L - 1.0f / U + (r2 - rj * rj + ri * ri) / (2.0f * r * ri * ri) * logf(L / U) * 0.5f / r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_born_sum(const double* __restrict__ pos,
+                                                    const float4* __restrict__ prm, int n,
+                                                    const uint32_t* __restrict__ nl,
+                                                    const int* __restrict__ cnt, int cap,
+                                                    double* __restrict__ sum,
+                                                    double* __restrict__ half) {
+    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (i >= n) return;
+    int lane = lane_id();
+    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+    float ri = prm[i].y;
+    int m = min(cnt[i], cap);
+    const uint32_t* row = nl + (size_t)i * cap;
+    double acc = 0.0;
+    for (int k = lane; k < m; k += kWave) {
+        int j = (int)(row[k] & kIdxMask);
+        double r2d = dist2_exact(pi, pos + 4 * j);
+        float r2 = (float)r2d, r = sqrtf(r2);
+        float rj = prm[j].y;
+        acc += (double)hct_f(r, r2, ri, rj);
+        atomicAdd(&sum[j], (double)hct_f(r, r2, rj, ri));
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) {
+        atomicAdd(&sum[i], acc);
+        half[i] = acc;
+    }
+}
+
+__global__ void k_born_finalize(int n, const double* __restrict__ rstar,
+                                const double* __restrict__ sum, double* __restrict__ a,
+                                float4* __restrict__ prm) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 1.0 / fmax(1.0 / rstar[i] - 0.5 * sum[i], 2.0);
+    a[i] = v;
+    prm[i].w = (float)v;
+}
+
+// ── fused pair energy ───────────────────────────────────────────────────────
+// pair_e / pair_e_diag, src/physics_engine.cpp:2148-2199.  Returns the four
+// terms DH, GB, LJ, hard-core in FP32.
+struct PairTerms {
+    float dh, gb, lj, hc;
+};
+
+__device__ __forceinline__ PairTerms pair_terms_f(float r2, float4 pi, float4 pj) {
+    PairTerms t;
+    float r = sqrtf(r2);
+    float qp = pi.x * pj.x;
+    float sig = pi.y + pj.y;
+    t.dh = ((float)(kCoulomb / kEpsWater) * qp / r) * expf(-(float)kKappa * r);
+    float D = pi.w * pj.w;
+    float fgb = sqrtf(r2 + D * expf(-r2 / (4.0f * D)));
+    t.gb = (float)kGbCoef * qp / fgb;
+    float eps = sqrtf(pi.z * pj.z);
+    float s2 = sig * sig / r2;
+    float s6 = s2 * s2 * s2;
+    t.lj = 4.0f * eps * (s6 * s6 - s6);
+    t.hc = 0.0f;
+    float rc = sig * (float)kHardFrac;
+    if (r < rc) {
+        float c2 = rc * rc / r2;
+        float c6 = c2 * c2 * c2;
+        float x = c6 * c6 - 1.0f;
+        t.hc = fminf((float)kHardScale * x * x, (float)kHardCap);
+    }
+    return t;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pair_energy(const double* __restrict__ pos,
+                                                       const float4* __restrict__ prm, int n,
+                                                       const uint32_t* __restrict__ nl,
+                                                       const int* __restrict__ cnt, int cap,
+                                                       int use_excl, double* __restrict__ part) {
+    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (i >= n) return;
+    int lane = lane_id();
+    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+    float4 qi = prm[i];
+    int m = min(cnt[i], cap);
+    const uint32_t* row = nl + (size_t)i * cap;
+    double dh = 0, gb = 0, lj = 0, hc = 0;
+    for (int k = lane; k < m; k += kWave) {
+        uint32_t e = row[k];
+        if (use_excl && (e & kExclBit)) continue;
+        int j = (int)(e & kIdxMask);
+        double r2d = dist2_exact(pi, pos + 4 * j);
+        if (r2d > kPairCut2) continue;
+        PairTerms t = pair_terms_f((float)r2d, qi, prm[j]);
+        dh += t.dh;
+        gb += t.gb;
+        lj += t.lj;
+        hc += t.hc;
+    }
+    dh = wave_sum(dh);
+    gb = wave_sum(gb);
+    lj = wave_sum(lj);
+    hc = wave_sum(hc);
+    if (lane == 0) {
+        part[5 * i + 0] = dh;
+        part[5 * i + 1] = gb;
+        part[5 * i + 2] = lj;
+        part[5 * i + 3] = hc;
+    }
+}
+
+// ── SASA ────────────────────────────────────────────────────────────────────
+// sasa_nonpolar, src/physics_engine.cpp:2068-2083: atom i is occluded only by
+// its half-list partners, folded in neighbour-list order.  One lane per atom.
+__global__ void k_sasa(const double* __restrict__ pos, const float4* __restrict__ prm, int n,
+                       const uint32_t* __restrict__ nl, const int* __restrict__ cnt, int cap,
+                       double* __restrict__ part) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+    double ri = (double)prm[i].y + kProbe;
+    double sa = 4.0 * M_PI * ri * ri;
+    const double k2 = 2.0 * M_PI * ri * ri / (2.0 * ri);
+    int m = min(cnt[i], cap);
+    const uint32_t* row = nl + (size_t)i * cap;
+    for (int k = 0; k < m; ++k) {
+        int j = (int)(row[k] & kIdxMask);
+        double rj = (double)prm[j].y + kProbe;
+        double dc = ri + rj;
+        double r2 = dist2_exact(pi, pos + 4 * j);
+        if (r2 >= dc * dc) continue;
+        double r = (double)sqrtf((float)r2);
+        if (r >= dc) continue;
+        sa -= fmin(sa * 0.85, k2 * (dc - r));
+    }
+    part[5 * i + 4] = kGammaSA * fmax(0.0, sa);
+}
+
+// ── dihedral / disulfide (FP64) ─────────────────────────────────────────────
+// dihedral_angle :1871-1885 (atan2 of the reference's non-IUPAC pair),
+// dihedral_e :1912-1921, ss_e :1965-1974.
+__device__ double dih_angle_d(const double* pos, int a, int b, int c, int d) {
+    double b1[3], b2[3], b3[3];
+    for (int k = 0; k < 3; ++k) {
+        b1[k] = pos[4 * a + k] - pos[4 * b + k];
+        b2[k] = pos[4 * c + k] - pos[4 * b + k];
+        b3[k] = pos[4 * d + k] - pos[4 * c + k];
+    }
+    double n1x = b1[1] * b2[2] - b1[2] * b2[1], n1y = b1[2] * b2[0] - b1[0] * b2[2],
+           n1z = b1[0] * b2[1] - b1[1] * b2[0];
+    double n2x = b2[1] * b3[2] - b2[2] * b3[1], n2y = b2[2] * b3[0] - b2[0] * b3[2],
+           n2z = b2[0] * b3[1] - b2[1] * b3[0];
+    double m1x = n1y * b2[2] - n1z * b2[1], m1y = n1z * b2[0] - n1x * b2[2],
+           m1z = n1x * b2[1] - n1y * b2[0];
+    return atan2(m1x * n2x + m1y * n2y + m1z * n2z, n1x * n2x + n1y * n2y + n1z * n2z);
+}
+
+__global__ void k_bonded(const double* __restrict__ pos, DevTopo t, double* __restrict__ rec) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < t.n_dih) {
+        double phi = dih_angle_d(pos, t.dih_a[k], t.dih_b[k], t.dih_c[k], t.dih_d[k]);
+        double e = 0.0;
+        for (int m = t.dih_toff[k]; m < t.dih_toff[k + 1]; ++m)
+            e += t.dih_v2[m] * (1.0 + cos((double)t.dih_n[m] * phi - t.dih_gamma[m]));
+        rec[k] = e;
+    } else if (k < t.n_dih + t.n_ss) {
+        int s = k - t.n_dih;
+        int i = t.ss_i[s], j = t.ss_j[s];
+        double dx = pos[4 * i] - pos[4 * j], dy = pos[4 * i + 1] - pos[4 * j + 1],
+               dz = pos[4 * i + 2] - pos[4 * j + 2];
+        double dr = sqrt(dx * dx + dy * dy + dz * dz) - kR0ss;
+        rec[k] = kKss * dr * dr;
+    }
+}
+
+// fixed-order FP64 reduction of the per-atom and per-record partials
+__global__ void k_reduce_terms(int n, const double* __restrict__ part, int n_dih, int n_ss,
+                               const double* __restrict__ rec, double* __restrict__ terms) {
+    __shared__ double buf[7][kBlock];
+    int t = threadIdx.x;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = t; i < n; i += kBlock)
+        for (int c = 0; c < 5; ++c) acc[c] += part[5 * i + c];
+    for (int k = t; k < n_dih; k += kBlock) acc[5] += rec[k];
+    for (int k = t; k < n_ss; k += kBlock) acc[6] += rec[n_dih + k];
+    for (int c = 0; c < 7; ++c) buf[c][t] = acc[c];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int c = 0; c < 7; ++c) buf[c][t] += buf[c][t + s];
+        __syncthreads();
+    }
+    if (t == 0) {
+        double sasa = kBetaSA + buf[4][0];
+        terms[ALMA_DH] = buf[0][0];
+        terms[ALMA_GB] = buf[1][0];
+        terms[ALMA_LJ] = buf[2][0];
+        terms[ALMA_HC] = buf[3][0];
+        terms[ALMA_SASA] = sasa;
+        terms[ALMA_DIH] = buf[5][0];
+        terms[ALMA_SS] = buf[6][0];
+        terms[ALMA_TOTAL] = sasa + buf[5][0] + buf[6][0] + buf[0][0] + buf[1][0] + buf[2][0] + buf[3][0];
+    }
+}
+
+inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
+
+}  // namespace
+
+hipError_t launch_build_nl(DevSys& s, hipStream_t st) {
+    const int n = s.n;
+    hipMemsetAsync(s.flags, 0, 2 * sizeof(int), st);
+    hipMemsetAsync(s.cell_cnt, 0, sizeof(int) * (s.max_cells + 1), st);
+    k_bbox<<<1, kBlock, 0, st>>>(s.pos, n, s.grid);
+    k_cell_assign<<<blocks_for(n, kBlock), kBlock, 0, st>>>(s.pos, n, s.grid, s.max_cells, s.cell_of,
+                                                            s.cell_cnt, s.flags);
+    k_cell_scan<<<1, 1024, 0, st>>>(s.grid, s.max_cells, s.cell_cnt, s.cell_start);
+    k_cell_place<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.cell_of, s.cell_start, s.cell_cnt,
+                                                           s.cell_tmp);
+    k_cell_rank<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.cell_of, s.cell_start, s.cell_tmp,
+                                                          s.cell_atoms);
+    k_nl_fill<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(
+        s.pos, n, s.grid, s.cell_of, s.cell_start, s.cell_atoms,
+        s.has_topo ? s.topo.excl_off : nullptr, s.has_topo ? s.topo.excl_idx : nullptr, s.cap, s.nl,
+        s.nl_cnt, s.flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_born(DevSys& s, hipStream_t st) {
+    const int n = s.n;
+    hipMemsetAsync(s.born_sum, 0, sizeof(double) * n, st);
+    k_born_sum<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s.pos, s.prm, n, s.nl, s.nl_cnt,
+                                                                 s.cap, s.born_sum, s.half_sum);
+    k_born_finalize<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.rstar, s.born_sum, s.a, s.prm);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(DevSys& s, hipStream_t st) {
+    const int n = s.n;
+    k_pair_energy<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s.pos, s.prm, n, s.nl, s.nl_cnt,
+                                                                    s.cap, s.has_topo ? 1 : 0, s.part);
+    k_sasa<<<blocks_for(n, kBlock), kBlock, 0, st>>>(s.pos, s.prm, n, s.nl, s.nl_cnt, s.cap, s.part);
+    int nrec = s.has_topo ? s.topo.n_dih + s.topo.n_ss : 0;
+    if (nrec > 0) k_bonded<<<blocks_for(nrec, kBlock), kBlock, 0, st>>>(s.pos, s.topo, s.rec);
+    k_reduce_terms<<<1, kBlock, 0, st>>>(n, s.part, s.has_topo ? s.topo.n_dih : 0,
+                                         s.has_topo ? s.topo.n_ss : 0, s.rec, s.terms);
+    return hipGetLastError();
+}
+
+}  // namespace alma

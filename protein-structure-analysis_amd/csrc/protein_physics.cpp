@@ -1,0 +1,512 @@
+// protein_physics.cpp -- the drop-in Python module `protein_physics`.
+//
+// Same classes, methods, argument names, defaults and error types as the
+// reference module (src/physics_engine.cpp:3692-3888), with PhysicsEngine's
+// arithmetic done by the gfx950 library behind include/alma.h.  The GIL is
+// released around every device call.  An engine owns one device context and,
+// like the reference engine, is not thread-safe: use one per thread.
+//
+// Added beyond the reference (all optional): a `seed` keyword on the MC
+// methods (the reference seeds from std::random_device), energy_terms(),
+// born_radii(), neighbor_list(), forces() and mc_trace().
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstdint>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/alma.h"
+#include "topology.h"
+
+namespace py = pybind11;
+using alma_host::BondKind;
+using alma_host::BondTopology;
+using alma_host::RotBond;
+
+namespace alma_py {
+
+// Particle, src/physics_engine.cpp:208-214
+struct Particle {
+    double x, y, z, charge, radius, epsilon;
+    bool is_water;
+    Particle(double x, double y, double z, double charge, double radius = 1.9, double epsilon = 0.1,
+             bool is_water = false)
+        : x(x), y(y), z(z), charge(charge), radius(radius), epsilon(epsilon), is_water(is_water) {}
+};
+
+using Parts = std::vector<Particle>;
+
+// flat CSR copy of a topology, owning its arrays
+struct Csr {
+    int n = 0;
+    std::vector<int32_t> excl_off, excl_idx, rb_i, rb_j, rb_kind, side_off, side_idx, dih_a, dih_b,
+        dih_c, dih_d, dih_toff, dih_n, ss_i, ss_j, cp_phi, cp_psi, csp_a, csp_b;
+    std::vector<double> rb_scale, dih_v2, dih_gamma;
+    alma_topo view() const {
+        alma_topo t{};
+        t.n_atoms = n;
+        t.excl_off = excl_off.data();
+        t.excl_idx = excl_idx.data();
+        t.n_rot = (int)rb_i.size();
+        t.rb_i = rb_i.data();
+        t.rb_j = rb_j.data();
+        t.rb_kind = rb_kind.data();
+        t.rb_scale = rb_scale.data();
+        t.side_off = side_off.data();
+        t.side_idx = side_idx.data();
+        t.n_dih = (int)dih_a.size();
+        t.dih_a = dih_a.data();
+        t.dih_b = dih_b.data();
+        t.dih_c = dih_c.data();
+        t.dih_d = dih_d.data();
+        t.dih_toff = dih_toff.data();
+        t.dih_v2 = dih_v2.data();
+        t.dih_n = dih_n.data();
+        t.dih_gamma = dih_gamma.data();
+        t.n_ss = (int)ss_i.size();
+        t.ss_i = ss_i.data();
+        t.ss_j = ss_j.data();
+        t.n_cp = (int)cp_phi.size();
+        t.cp_phi = cp_phi.data();
+        t.cp_psi = cp_psi.data();
+        t.n_csp = (int)csp_a.size();
+        t.csp_a = csp_a.data();
+        t.csp_b = csp_b.data();
+        return t;
+    }
+};
+
+void csr_lists(const std::vector<std::vector<int>>& v, std::vector<int32_t>& off, std::vector<int32_t>& idx) {
+    off.assign(1, 0);
+    idx.clear();
+    for (const auto& row : v) {
+        idx.insert(idx.end(), row.begin(), row.end());
+        off.push_back((int32_t)idx.size());
+    }
+}
+
+Csr csr_of(const BondTopology& t) {
+    Csr c;
+    c.n = t.N;
+    csr_lists(t.excl, c.excl_off, c.excl_idx);
+    if (c.excl_off.size() != (size_t)t.N + 1) c.excl_off.assign((size_t)t.N + 1, 0);
+    for (const auto& rb : t.rot_bonds) {
+        c.rb_i.push_back(rb.i);
+        c.rb_j.push_back(rb.j);
+        c.rb_kind.push_back((int)rb.kind);
+    }
+    c.rb_scale = t.rot_bond_scale;
+    csr_lists(t.rot_bond_sides, c.side_off, c.side_idx);
+    c.dih_toff.push_back(0);
+    for (const auto& d : t.dihedrals) {
+        c.dih_a.push_back(d.a);
+        c.dih_b.push_back(d.b);
+        c.dih_c.push_back(d.c);
+        c.dih_d.push_back(d.d);
+        for (const auto& tm : d.terms) {
+            c.dih_v2.push_back(tm.V2);
+            c.dih_n.push_back(tm.n);
+            c.dih_gamma.push_back(tm.gamma);
+        }
+        c.dih_toff.push_back((int32_t)c.dih_v2.size());
+    }
+    for (const auto& p : t.disulfide_pairs) {
+        c.ss_i.push_back(p.first);
+        c.ss_j.push_back(p.second);
+    }
+    for (const auto& p : t.concerted_pairs) {
+        c.cp_phi.push_back(p.first);
+        c.cp_psi.push_back(p.second);
+    }
+    for (const auto& p : t.concerted_sidechain_pairs) {
+        c.csp_a.push_back(p.first);
+        c.csp_b.push_back(p.second);
+    }
+    return c;
+}
+
+// Any object exposing the reference BondTopology attribute protocol
+// (src/physics_engine.cpp:3729-3841), e.g. the reference CPU module's
+// topology: the GPU module of the reference reads the same attributes
+// (src/physics_engine_cuda.cu:341-374).
+Csr csr_from_object(const py::object& o) {
+    if (py::isinstance<BondTopology>(o)) return csr_of(o.cast<const BondTopology&>());
+    Csr c;
+    c.n = o.attr("num_atoms").cast<int>();
+    csr_lists(o.attr("excl").cast<std::vector<std::vector<int>>>(), c.excl_off, c.excl_idx);
+    if (c.excl_off.size() != (size_t)c.n + 1) c.excl_off.assign((size_t)c.n + 1, 0);
+    c.rb_i = o.attr("rb_atom_i").cast<std::vector<int32_t>>();
+    c.rb_j = o.attr("rb_atom_j").cast<std::vector<int32_t>>();
+    c.rb_kind = o.attr("rb_kind").cast<std::vector<int32_t>>();
+    c.rb_scale = o.attr("rot_bond_scale").cast<std::vector<double>>();
+    csr_lists(o.attr("rot_bond_sides").cast<std::vector<std::vector<int>>>(), c.side_off, c.side_idx);
+    c.dih_a = o.attr("dih_a").cast<std::vector<int32_t>>();
+    c.dih_b = o.attr("dih_b").cast<std::vector<int32_t>>();
+    c.dih_c = o.attr("dih_c").cast<std::vector<int32_t>>();
+    c.dih_d = o.attr("dih_d").cast<std::vector<int32_t>>();
+    c.dih_toff = o.attr("dih_term_offsets").cast<std::vector<int32_t>>();
+    c.dih_v2 = o.attr("dih_term_v2").cast<std::vector<double>>();
+    c.dih_n = o.attr("dih_term_n").cast<std::vector<int32_t>>();
+    c.dih_gamma = o.attr("dih_term_gamma").cast<std::vector<double>>();
+    for (auto& p : o.attr("disulfide_pairs").cast<std::vector<std::pair<int, int>>>()) {
+        c.ss_i.push_back(p.first);
+        c.ss_j.push_back(p.second);
+    }
+    for (auto& p : o.attr("concerted_pairs").cast<std::vector<std::pair<int, int>>>()) {
+        c.cp_phi.push_back(p.first);
+        c.cp_psi.push_back(p.second);
+    }
+    if (py::hasattr(o, "concerted_sidechain_pairs"))
+        for (auto& p : o.attr("concerted_sidechain_pairs").cast<std::vector<std::pair<int, int>>>()) {
+            c.csp_a.push_back(p.first);
+            c.csp_b.push_back(p.second);
+        }
+    return c;
+}
+
+class Engine {
+public:
+    Engine() {
+        alma_status s = alma_create(0, &ctx_);
+        if (s != ALMA_OK)
+            throw std::runtime_error(s == ALMA_ENODEV ? "no GPU device available for protein_physics"
+                                                      : "failed to create the GPU context");
+    }
+    ~Engine() { alma_destroy(ctx_); }
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    void check(alma_status s) const {
+        if (s == ALMA_OK) return;
+        std::string m = alma_last_error(ctx_);
+        if (s == ALMA_EINVAL) throw std::invalid_argument(m);
+        throw std::runtime_error(m);
+    }
+
+    void load(const Parts& p, const Csr* topo) {
+        size_t n = p.size();
+        xyz_.resize(3 * n);
+        q_.resize(n);
+        r_.resize(n);
+        e_.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            xyz_[3 * i] = p[i].x;
+            xyz_[3 * i + 1] = p[i].y;
+            xyz_[3 * i + 2] = p[i].z;
+            q_[i] = p[i].charge;
+            r_[i] = p[i].radius;
+            e_[i] = p[i].epsilon;
+        }
+        alma_topo view;
+        if (topo) view = topo->view();
+        check(alma_set_system(ctx_, (int32_t)n, xyz_.data(), q_.data(), r_.data(), e_.data(),
+                              topo ? &view : nullptr));
+    }
+
+    std::vector<double> terms(const Parts& p, const Csr* topo) {
+        load(p, topo);
+        std::vector<double> t(ALMA_NTERMS, 0.0);
+        check(alma_energy(ctx_, t.data()));
+        return t;
+    }
+
+    Parts with_coords(const Parts& tmpl, const double* xyz) const {
+        Parts out = tmpl;
+        for (size_t i = 0; i < out.size(); ++i) {
+            out[i].x = xyz[3 * i];
+            out[i].y = xyz[3 * i + 1];
+            out[i].z = xyz[3 * i + 2];
+        }
+        return out;
+    }
+
+    struct McOut {
+        std::vector<double> xyz;
+        std::vector<alma_mc_stats> stats;
+        std::vector<double> snap_xyz, snap_e;
+    };
+
+    McOut mc(const Parts& init, const Csr& topo, int mode, int nchains, int steps, int snap_every,
+             double T, double max_angle, double mala_step, uint32_t seed, bool trace) {
+        load(init, &topo);
+        alma_mc_params p{};
+        p.mode = mode;
+        p.n_chains = nchains;
+        p.steps = steps;
+        p.snapshot_every = snap_every;
+        p.temperature = T;
+        p.max_angle = max_angle;
+        p.mala_step = mala_step;
+        p.seed = seed;
+        p.trace = trace ? 1 : 0;
+        McOut o;
+        size_t n = init.size();
+        o.xyz.resize((size_t)nchains * 3 * n);
+        o.stats.resize(nchains);
+        int nsnap = snap_every > 0 ? steps / snap_every : 0;
+        if (nsnap) {
+            o.snap_xyz.resize((size_t)nchains * nsnap * 3 * n);
+            o.snap_e.resize((size_t)nchains * nsnap);
+        }
+        check(alma_mc_run(ctx_, &p, o.xyz.data(), o.stats.data(), nsnap ? o.snap_xyz.data() : nullptr,
+                          nsnap ? o.snap_e.data() : nullptr));
+        return o;
+    }
+
+    alma_ctx* ctx() { return ctx_; }
+
+private:
+    alma_ctx* ctx_ = nullptr;
+    std::vector<double> xyz_, q_, r_, e_;
+};
+
+uint32_t draw_seed(const py::object& seed) {
+    if (!seed.is_none()) return seed.cast<uint32_t>();
+    return std::random_device{}();
+}
+
+void check_mc_args(const Parts& init, const Csr& t, int a, int b, const char* what) {
+    if (init.empty()) throw std::invalid_argument("initial_state empty");
+    if (a <= 0 || b <= 0) throw std::invalid_argument(what);
+    if (t.rb_i.empty()) throw std::invalid_argument("topology has no rotatable bonds");
+    if (t.n != (int)init.size()) throw std::invalid_argument("topology atom count differs from the state");
+}
+
+}  // namespace alma_py
+
+using namespace alma_py;
+
+PYBIND11_MODULE(protein_physics, m) {
+    m.doc() = "MI355X implicit-solvent engine (HIP neighbour list, HCT-GB, SASA, device MC)";
+
+    py::class_<Particle>(m, "Particle", py::module_local())
+        .def(py::init<double, double, double, double, double, double, bool>(), py::arg("x"), py::arg("y"),
+             py::arg("z"), py::arg("charge"), py::arg("radius") = 1.9, py::arg("epsilon") = 0.1,
+             py::arg("is_water") = false)
+        .def_readwrite("x", &Particle::x)
+        .def_readwrite("y", &Particle::y)
+        .def_readwrite("z", &Particle::z)
+        .def_readwrite("charge", &Particle::charge)
+        .def_readwrite("radius", &Particle::radius)
+        .def_readwrite("epsilon", &Particle::epsilon)
+        .def_readwrite("is_water", &Particle::is_water);
+
+    py::enum_<BondKind>(m, "BondKind", py::module_local())
+        .value("BACKBONE_PHI", BondKind::BACKBONE_PHI)
+        .value("BACKBONE_PSI", BondKind::BACKBONE_PSI)
+        .value("SIDECHAIN", BondKind::SIDECHAIN)
+        .value("FIXED", BondKind::FIXED);
+
+    py::class_<RotBond>(m, "RotBond", py::module_local())
+        .def_readonly("i", &RotBond::i)
+        .def_readonly("j", &RotBond::j)
+        .def_readonly("kind", &RotBond::kind);
+
+    py::class_<BondTopology>(m, "BondTopology", py::module_local())
+        .def(py::init<>())
+        .def("build", &BondTopology::build, py::arg("resnames"), py::arg("atomnames"), py::arg("res_idx"))
+        .def("bonded", &BondTopology::bonded, py::arg("i"), py::arg("j"))
+        .def("j_side", &BondTopology::j_side, py::arg("bond_i"), py::arg("bond_j"))
+        .def("is_excluded", &BondTopology::is_excluded, py::arg("i"), py::arg("j"))
+        .def_readonly("adj", &BondTopology::adj)
+        .def_readonly("bonds", &BondTopology::bonds)
+        .def_readonly("rot_bonds", &BondTopology::rot_bonds)
+        .def_readonly("rot_bond_sides", &BondTopology::rot_bond_sides)
+        .def_readonly("excl", &BondTopology::excl)
+        .def_readonly("rot_bond_scale", &BondTopology::rot_bond_scale)
+        .def_readonly("disulfide_pairs", &BondTopology::disulfide_pairs)
+        .def_readonly("concerted_pairs", &BondTopology::concerted_pairs)
+        .def_readonly("concerted_sidechain_pairs", &BondTopology::concerted_sidechain_pairs)
+        .def(
+            "identify_concerted_sidechain_pairs",
+            [](BondTopology& t, const Parts& p, double cutoff) {
+                std::vector<alma_host::XYZ> pos(p.size());
+                for (size_t i = 0; i < p.size(); ++i) pos[i] = {p[i].x, p[i].y, p[i].z};
+                t.identify_concerted_sidechain_pairs(pos, cutoff);
+            },
+            py::arg("init_coords"), py::arg("cutoff") = 6.0)
+        .def("add_disulfide", &BondTopology::add_disulfide, py::arg("atom_i"), py::arg("atom_j"))
+        .def_property_readonly("rb_atom_i", [](const BondTopology& t) { return csr_of(t).rb_i; })
+        .def_property_readonly("rb_atom_j", [](const BondTopology& t) { return csr_of(t).rb_j; })
+        .def_property_readonly("rb_kind", [](const BondTopology& t) { return csr_of(t).rb_kind; })
+        .def_property_readonly("dih_a", [](const BondTopology& t) { return csr_of(t).dih_a; })
+        .def_property_readonly("dih_b", [](const BondTopology& t) { return csr_of(t).dih_b; })
+        .def_property_readonly("dih_c", [](const BondTopology& t) { return csr_of(t).dih_c; })
+        .def_property_readonly("dih_d", [](const BondTopology& t) { return csr_of(t).dih_d; })
+        .def_property_readonly("dih_term_offsets", [](const BondTopology& t) { return csr_of(t).dih_toff; })
+        .def_property_readonly("dih_term_v2", [](const BondTopology& t) { return csr_of(t).dih_v2; })
+        .def_property_readonly("dih_term_n", [](const BondTopology& t) { return csr_of(t).dih_n; })
+        .def_property_readonly("dih_term_gamma", [](const BondTopology& t) { return csr_of(t).dih_gamma; })
+        .def_property_readonly("num_dihedrals", [](const BondTopology& t) { return (int)t.dihedrals.size(); })
+        .def_property_readonly("num_atoms", [](const BondTopology& t) { return t.N; })
+        .def_property_readonly("num_bonds", [](const BondTopology& t) { return (int)t.bonds.size(); })
+        .def_property_readonly("num_rot_bonds", [](const BondTopology& t) { return (int)t.rot_bonds.size(); })
+        .def_property_readonly("num_concerted_pairs",
+                               [](const BondTopology& t) { return (int)t.concerted_pairs.size(); })
+        .def_property_readonly("num_disulfide_pairs",
+                               [](const BondTopology& t) { return (int)t.disulfide_pairs.size(); });
+
+    py::class_<Engine>(m, "PhysicsEngine", py::module_local())
+        .def(py::init<>())
+        .def(
+            "calculate_potential",
+            [](Engine& self, const Parts& particles, py::object topo) -> double {
+                Csr c;
+                bool has = !topo.is_none();
+                if (has) c = csr_from_object(topo);
+                py::gil_scoped_release rel;
+                return self.terms(particles, has ? &c : nullptr)[ALMA_TOTAL];
+            },
+            py::arg("particles"), py::arg("topology") = py::none())
+        .def(
+            "energy_terms",
+            [](Engine& self, const Parts& particles, py::object topo) {
+                Csr c;
+                bool has = !topo.is_none();
+                if (has) c = csr_from_object(topo);
+                std::vector<double> t;
+                {
+                    py::gil_scoped_release rel;
+                    t = self.terms(particles, has ? &c : nullptr);
+                }
+                static const char* names[] = {"DH", "GB", "LJ", "HC", "SASA", "DIH", "SS", "TOTAL"};
+                py::dict d;
+                for (int k = 0; k < ALMA_NTERMS; ++k) d[names[k]] = t[k];
+                return d;
+            },
+            py::arg("particles"), py::arg("topology") = py::none())
+        .def(
+            "born_radii",
+            [](Engine& self, const Parts& particles) {
+                std::vector<double> a(particles.size());
+                py::gil_scoped_release rel;
+                self.load(particles, nullptr);
+                if (!particles.empty()) self.check(alma_born_radii(self.ctx(), a.data()));
+                return a;
+            },
+            py::arg("particles"))
+        .def(
+            "neighbor_list",
+            [](Engine& self, const Parts& particles) {
+                std::vector<int32_t> offs(particles.size() + 1, 0), js;
+                {
+                    py::gil_scoped_release rel;
+                    self.load(particles, nullptr);
+                    int64_t np = 0;
+                    self.check(alma_nlist(self.ctx(), offs.data(), nullptr, 0, &np));
+                    js.resize((size_t)np);
+                    self.check(alma_nlist(self.ctx(), offs.data(), js.data(), np, &np));
+                }
+                return std::make_pair(offs, js);
+            },
+            py::arg("particles"))
+        .def(
+            "generate_ensemble",
+            [](Engine& self, const Parts& init, py::object topo, int ncand, int steps, double T,
+               double max_angle, py::object seed) {
+                Csr c = csr_from_object(topo);
+                check_mc_args(init, c, ncand, steps, "ncand/steps must be positive");
+                uint32_t s = draw_seed(seed);
+                std::vector<Parts> ens;
+                {
+                    py::gil_scoped_release rel;
+                    auto o = self.mc(init, c, ALMA_MC_ENSEMBLE, ncand, steps, 0, T, max_angle, 0.0, s, false);
+                    for (int k = 0; k < ncand; ++k)
+                        ens.push_back(self.with_coords(init, o.xyz.data() + (size_t)k * 3 * init.size()));
+                }
+                return ens;
+            },
+            py::arg("initial_state"), py::arg("topology"), py::arg("n_candidates"), py::arg("steps_per_cand"),
+            py::arg("temperature") = 0.6, py::arg("max_angle") = 0.12, py::arg("seed") = py::none())
+        .def(
+            "run_landscape_trajectory",
+            [](Engine& self, const Parts& init, py::object topo, int nsnap, int per, double T,
+               double max_angle, py::object seed) {
+                Csr c = csr_from_object(topo);
+                check_mc_args(init, c, nsnap, per, "n_snapshots/steps_per_snapshot must be positive");
+                uint32_t s = draw_seed(seed);
+                std::vector<Parts> snaps;
+                std::vector<double> es;
+                {
+                    py::gil_scoped_release rel;
+                    auto o = self.mc(init, c, ALMA_MC_LANDSCAPE, 1, nsnap * per, per, T, max_angle, 0.0, s, false);
+                    for (int k = 0; k < nsnap; ++k)
+                        snaps.push_back(self.with_coords(init, o.snap_xyz.data() + (size_t)k * 3 * init.size()));
+                    es = o.snap_e;
+                }
+                return std::make_pair(snaps, es);
+            },
+            py::arg("initial_state"), py::arg("topology"), py::arg("n_snapshots"),
+            py::arg("steps_per_snapshot"), py::arg("temperature") = 0.6, py::arg("max_angle") = 0.12,
+            py::arg("seed") = py::none())
+        .def(
+            "run_landscape_segment",
+            [](Engine& self, const Parts& init, py::object topo, int nsnap, int per, double T,
+               double max_angle, py::object seed) {
+                Csr c = csr_from_object(topo);
+                check_mc_args(init, c, nsnap, per, "n_snapshots/steps_per_snapshot must be positive");
+                uint32_t s = draw_seed(seed);
+                std::vector<Parts> snaps;
+                std::vector<double> es;
+                double cur_max;
+                {
+                    py::gil_scoped_release rel;
+                    auto o = self.mc(init, c, ALMA_MC_LANDSCAPE, 1, nsnap * per, per, T, max_angle, 0.0, s, false);
+                    for (int k = 0; k < nsnap; ++k)
+                        snaps.push_back(self.with_coords(init, o.snap_xyz.data() + (size_t)k * 3 * init.size()));
+                    es = o.snap_e;
+                    cur_max = o.stats[0].cur_max;
+                }
+                return std::make_tuple(snaps, es, cur_max);
+            },
+            py::arg("initial_state"), py::arg("topology"), py::arg("n_snapshots"),
+            py::arg("steps_per_snapshot"), py::arg("temperature") = 0.6, py::arg("max_angle") = 0.12,
+            py::arg("seed") = py::none())
+        .def(
+            "lowest_energy_structure",
+            [](Engine& self, const std::vector<Parts>& ens) {
+                // calculate_potential without topology, as the reference does
+                // (src/physics_engine.cpp:3672-3676)
+                if (ens.empty()) throw std::invalid_argument("ensemble empty");
+                py::gil_scoped_release rel;
+                size_t best = 0;
+                double be = 0.0;
+                for (size_t k = 0; k < ens.size(); ++k) {
+                    double e = self.terms(ens[k], nullptr)[ALMA_TOTAL];
+                    if (k == 0 || e < be) {
+                        be = e;
+                        best = k;
+                    }
+                }
+                return ens[best];
+            },
+            py::arg("ensemble"))
+        .def(
+            "forces",
+            [](Engine& self, const Parts& particles, py::object topo) {
+                Csr c;
+                bool has = !topo.is_none();
+                if (has) c = csr_from_object(topo);
+                double e = 0.0;
+                std::vector<double> g(3 * particles.size()), gt(has ? c.rb_i.size() : 0);
+                {
+                    py::gil_scoped_release rel;
+                    self.load(particles, has ? &c : nullptr);
+                    self.check(alma_forces(self.ctx(), &e, g.data(), gt.empty() ? nullptr : gt.data()));
+                }
+                return py::make_tuple(e, g, gt);
+            },
+            py::arg("particles"), py::arg("topology") = py::none())
+        .def("num_threads", [](Engine&) { return 1; })
+        .def_static("device_name", []() {
+            alma_ctx* c = nullptr;
+            if (alma_create(0, &c) != ALMA_OK) throw std::runtime_error("no GPU device available");
+            char buf[256];
+            alma_device_name(c, buf, sizeof(buf));
+            alma_destroy(c);
+            return std::string(buf);
+        });
+}

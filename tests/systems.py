@@ -1,0 +1,48 @@
+"""Seeded test systems shared by the CPU and GPU tests (cached per process)."""
+import functools
+
+import numpy as np
+
+from alma import synth
+
+
+@functools.lru_cache(maxsize=None)
+def chain(n_res, seed):
+    return synth.make_chain(n_res, seed)
+
+
+@functools.lru_cache(maxsize=None)
+def two_chains_ss():
+    s = synth.concat([synth.make_chain(18, 11, seq=["CYS"] + ["ALA"] * 8 + ["GLY"] * 9),
+                      synth.make_chain(14, 12, seq=["GLY"] * 6 + ["CYS"] + ["SER"] * 7)], gap=2)
+    return s
+
+
+def bridge():
+    """tests/bridge_test.py:83-85 of the reference: three particles."""
+    return {"xyz": np.array([[0.0, 0, 0], [5.0, 0, 0], [10.0, 0, 0]]),
+            "charge": np.array([1.0, -1.0, 0.5]), "rstar": np.array([1.9, 1.9, 1.9]),
+            "eps": np.array([0.1, 0.1, 0.1])}
+
+
+def topology(sysd, ss_pairs=()):
+    """Product BondTopology (pybind drop-in) for a system dict."""
+    import protein_physics as PP
+    t = PP.BondTopology()
+    t.build(list(sysd["resnames"]), list(sysd["atomnames"]), [int(v) for v in sysd["res_idx"]])
+    for i, j in ss_pairs:
+        t.add_disulfide(int(i), int(j))
+    return t
+
+
+def particles(sysd, mod=None):
+    if mod is None:
+        import protein_physics as mod
+    x = sysd["xyz"]
+    return [mod.Particle(float(x[k, 0]), float(x[k, 1]), float(x[k, 2]), float(sysd["charge"][k]),
+                         float(sysd["rstar"][k]), float(sysd["eps"][k])) for k in range(len(x))]
+
+
+def csr(t):
+    from oracle.oracle import topo_to_csr
+    return topo_to_csr(t)
