@@ -117,9 +117,11 @@ typedef struct {
 alma_status alma_mc_run(alma_ctx* ctx, const alma_mc_params* p, double* xyz_out,
                         alma_mc_stats* stats, double* snap_xyz, double* snap_e);
 /* Trace of chain 0 of the last alma_mc_run with trace != 0 (length steps):
- * move kind, bond / pair index, delta, dE, accepted. */
+ * move kind, torsion bond index, delta, dE, accepted, and dE split into
+ * (cross pairs, SASA, dihedral, SS) as 4 doubles per step.  Any pointer may
+ * be NULL. */
 alma_status alma_mc_trace(alma_ctx* ctx, int32_t* kind, int32_t* bond, double* delta,
-                          double* dE, int32_t* accepted);
+                          double* dE, int32_t* accepted, double* dE_parts);
 
 /* ---- forces (no reference counterpart) ---- */
 /* E, dE/dx (3n) with the Born-radius chain rule, dE/dtheta per rotatable

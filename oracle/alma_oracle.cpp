@@ -387,6 +387,7 @@ struct Chain {
     // last-move record for tracing
     int last_kind = -1, last_bond = -1;
     double last_delta = 0, last_dE = 0, last_l1 = 0;
+    double last_parts[4] = {0, 0, 0, 0};
     int last_acc = 0;
 
     Chain(const Sys& sys, const double* x0, const orc_topo* topo, uint32_t seed, double max_angle)
@@ -514,6 +515,10 @@ struct Chain {
         Terms nw = terms(want_items ? &in : nullptr);
         double dE = (nw.cross - o.cross) + (nw.sasa - o.sasa) + (nw.dih - o.dih) + (nw.ss - o.ss);
         last_dE = dE;
+        last_parts[0] = nw.cross - o.cross;
+        last_parts[1] = nw.sasa - o.sasa;
+        last_parts[2] = nw.dih - o.dih;
+        last_parts[3] = nw.ss - o.ss;
         last_l1 = want_items ? l1_of(io, in) : 0.0;
         bool acc = metropolis(dE, T);
         if (acc)
@@ -619,6 +624,10 @@ struct Chain {
         Terms nw = terms(want_items ? &in : nullptr);
         double dE = (nw.cross - o.cross) + (nw.sasa - o.sasa) + (nw.dih - o.dih) + (nw.ss - o.ss);
         last_dE = dE;
+        last_parts[0] = nw.cross - o.cross;
+        last_parts[1] = nw.sasa - o.sasa;
+        last_parts[2] = nw.dih - o.dih;
+        last_parts[3] = nw.ss - o.ss;
         last_l1 = want_items ? l1_of(io, in) : 0.0;
         bool acc = metropolis(dE, T);
         if (acc)
@@ -689,6 +698,7 @@ struct Chain {
         last_acc = 0;
         last_dE = 0;
         last_l1 = 0;
+        for (double& v : last_parts) v = 0;
         std::pair<bool, bool> r;
         if (mode == ORC_MODE_TORSION30) {
             r = torsion(T, true);
@@ -993,6 +1003,8 @@ int orc_mc_run(int n, double* xyz, const double* q, const double* rstar, const d
             if (tr->dE_l1) tr->dE_l1[k] = ch.last_l1;
             if (tr->accepted) tr->accepted[k] = ch.last_acc;
             if (tr->words_before) tr->words_before[k] = before;
+            if (tr->parts)
+                for (int q = 0; q < 4; ++q) tr->parts[4 * k + q] = ch.last_parts[q];
         }
         if (p->snapshot_every > 0 && (k + 1) % p->snapshot_every == 0) {
             if (snap_xyz) std::copy(ch.x.begin(), ch.x.end(), snap_xyz + (size_t)snap * 3 * n);

@@ -84,6 +84,7 @@ typedef struct {
     double* dE_l1;          /* sum over terms of |per-item change|             */
     int* accepted;
     int64_t* words_before;  /* RNG cursor before the step                      */
+    double* parts;          /* 4 per step: cross pairs, SASA, dihedral, SS     */
 } orc_mc_trace;
 
 /* Run one chain.  xyz is updated in place.  snap_xyz (n_snap*3n) and

@@ -43,7 +43,7 @@ class McStats(C.Structure):
 
 class McTrace(C.Structure):
     _fields_ = [("kind", _I), ("bond", _I), ("delta", _D), ("dE", _D), ("dE_l1", _D),
-                ("accepted", _I), ("words_before", C.POINTER(C.c_int64))]
+                ("accepted", _I), ("words_before", C.POINTER(C.c_int64)), ("parts", _D)]
 
 
 def lib():
@@ -174,11 +174,12 @@ def mc_run(sysd, topo_csr, steps, temperature=0.6, max_angle=0.12, seed=5489,
                   "delta": np.zeros(steps), "dE": np.zeros(steps),
                   "dE_l1": np.zeros(steps) if l1 else None,
                   "accepted": np.zeros(steps, np.int32),
-                  "words_before": np.zeros(steps, np.int64)}
+                  "words_before": np.zeros(steps, np.int64), "parts": np.zeros((steps, 4))}
         tr = McTrace(_p(arrays["kind"], C.c_int), _p(arrays["bond"], C.c_int),
                      _p(arrays["delta"], C.c_double), _p(arrays["dE"], C.c_double),
                      _p(arrays["dE_l1"], C.c_double), _p(arrays["accepted"], C.c_int),
-                     arrays["words_before"].ctypes.data_as(C.POINTER(C.c_int64)))
+                     arrays["words_before"].ctypes.data_as(C.POINTER(C.c_int64)),
+                     _p(arrays["parts"], C.c_double))
     nsnap = steps // snapshot_every if snapshot_every else 0
     snap_x = np.zeros((max(nsnap, 1), n, 3))
     snap_e = np.zeros(max(nsnap, 1))

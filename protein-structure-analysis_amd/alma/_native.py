@@ -67,7 +67,7 @@ def lib():
         L.alma_nlist.argtypes = [P, _I, _I, C.c_int64, C.POINTER(C.c_int64)]
         L.alma_born_radii.argtypes = [P, _D]
         L.alma_mc_run.argtypes = [P, C.POINTER(McParams), _D, C.POINTER(McStats), _D, _D]
-        L.alma_mc_trace.argtypes = [P, _I, _I, _D, _D, _I]
+        L.alma_mc_trace.argtypes = [P, _I, _I, _D, _D, _I, _D]
         L.alma_forces.argtypes = [P, _D, _D, _D]
         _LIB = L
     return _LIB
@@ -203,10 +203,11 @@ class Context:
         if trace:
             tr = {"kind": np.zeros(steps, np.int32), "bond": np.zeros(steps, np.int32),
                   "delta": np.zeros(steps), "dE": np.zeros(steps),
-                  "accepted": np.zeros(steps, np.int32)}
+                  "accepted": np.zeros(steps, np.int32), "parts": np.zeros((steps, 4))}
             self._check(lib().alma_mc_trace(self._h, _p(tr["kind"], C.c_int32),
                                             _p(tr["bond"], C.c_int32), _p(tr["delta"], C.c_double),
-                                            _p(tr["dE"], C.c_double), _p(tr["accepted"], C.c_int32)))
+                                            _p(tr["dE"], C.c_double), _p(tr["accepted"], C.c_int32),
+                                            _p(tr["parts"], C.c_double)))
             out["trace"] = tr
         return out
 

@@ -81,6 +81,7 @@ void free_sys(DevSys& s) {
     dev_free(s.pos);
     dev_free(s.prm);
     dev_free(s.rstar);
+    dev_free(s.eps64);
     dev_free(s.born_sum);
     dev_free(s.half_sum);
     dev_free(s.a);
@@ -257,6 +258,298 @@ void evaluate(alma_ctx* ctx, double* terms) {
     throw Fail{ALMA_ERUNTIME, "neighbour list capacity could not be sized"};
 }
 
+// ─────────────────────────────── MC driver ─────────────────────────────────
+// mt19937 on the host feeds each chain a buffer of raw output words; the
+// kernel consumes them with the libstdc++ distribution rules, so a chain's
+// draws equal std::mt19937(seed + chain) consumed by the reference loop.
+struct HostMT {
+    uint32_t s[624];
+    int idx = 624;
+    explicit HostMT(uint32_t seed = 5489u) {
+        s[0] = seed;
+        for (int i = 1; i < 624; ++i) s[i] = 1812433253u * (s[i - 1] ^ (s[i - 1] >> 30)) + (uint32_t)i;
+    }
+    uint32_t operator()() {
+        if (idx >= 624) {
+            for (int i = 0; i < 624; ++i) {
+                uint32_t y = (s[i] & 0x80000000u) | (s[(i + 1) % 624] & 0x7fffffffu);
+                uint32_t v = s[(i + 397) % 624] ^ (y >> 1);
+                if (y & 1u) v ^= 0x9908b0dfu;
+                s[i] = v;
+            }
+            idx = 0;
+        }
+        uint32_t y = s[idx++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+};
+
+void free_chains(ChainBatch& b) {
+    dev_free(b.pos);
+    dev_free(b.posn);
+    dev_free(b.ref);
+    dev_free(b.a);
+    dev_free(b.anew);
+    dev_free(b.half);
+    dev_free(b.dhalf);
+    dev_free(b.born_sum);
+    dev_free(b.part);
+    dev_free(b.rec);
+    dev_free(b.flag);
+    dev_free(b.nlh);
+    dev_free(b.nlf);
+    dev_free(b.nls);
+    dev_free(b.nlh_cnt);
+    dev_free(b.nlf_cnt);
+    dev_free(b.nls_cnt);
+    dev_free(b.cell_of);
+    dev_free(b.cell_cnt);
+    dev_free(b.cell_start);
+    dev_free(b.cell_atoms);
+    dev_free(b.cell_tmp);
+    dev_free(b.grid);
+    dev_free(b.flags);
+    dev_free(b.words);
+    dev_free(b.st);
+    dev_free(b.list);
+    dev_free(b.tr_kind);
+    dev_free(b.tr_bond);
+    dev_free(b.tr_delta);
+    dev_free(b.tr_dE);
+    dev_free(b.tr_acc);
+    dev_free(b.tr_parts);
+    b = ChainBatch{};
+}
+
+void alloc_rows(ChainBatch& b) {
+    size_t rows = (size_t)b.nch * b.n;
+    dev_alloc(&b.nlh, rows * b.cap_h);
+    dev_alloc(&b.nlf, rows * b.cap_f);
+    dev_alloc(&b.nls, rows * b.cap_s);
+}
+
+void alloc_chains(ChainBatch& b, const DevSys& s, int nch, int trace_steps) {
+    free_chains(b);
+    b.n = s.n;
+    b.nch = nch;
+    b.max_cells = s.max_cells;
+    b.wcap = 16384;
+    size_t N = (size_t)nch * s.n;
+    dev_alloc(&b.pos, 4 * N);
+    dev_alloc(&b.posn, 4 * N);
+    dev_alloc(&b.ref, 4 * N);
+    dev_alloc(&b.a, N);
+    dev_alloc(&b.anew, N);
+    dev_alloc(&b.half, N);
+    dev_alloc(&b.dhalf, N);
+    dev_alloc(&b.born_sum, N);
+    dev_alloc(&b.part, 5 * N);
+    dev_alloc(&b.flag, N);
+    dev_alloc(&b.nlh_cnt, N);
+    dev_alloc(&b.nlf_cnt, N);
+    dev_alloc(&b.nls_cnt, N);
+    dev_alloc(&b.cell_of, N);
+    dev_alloc(&b.cell_cnt, (size_t)nch * (b.max_cells + 1));
+    dev_alloc(&b.cell_start, (size_t)nch * (b.max_cells + 1));
+    dev_alloc(&b.cell_atoms, N);
+    dev_alloc(&b.cell_tmp, N);
+    dev_alloc(&b.grid, nch);
+    dev_alloc(&b.flags, 4 * (size_t)nch);
+    dev_alloc(&b.words, (size_t)nch * b.wcap);
+    dev_alloc(&b.st, nch);
+    dev_alloc(&b.list, nch);
+    HIP_TRY(hipMemset(b.dhalf, 0, sizeof(double) * N));
+    HIP_TRY(hipMemset(b.flag, 0, sizeof(int) * N));
+    if (trace_steps > 0) {
+        b.trace_cap = trace_steps;
+        dev_alloc(&b.tr_kind, trace_steps);
+        dev_alloc(&b.tr_bond, trace_steps);
+        dev_alloc(&b.tr_delta, trace_steps);
+        dev_alloc(&b.tr_dE, trace_steps);
+        dev_alloc(&b.tr_acc, trace_steps);
+        dev_alloc(&b.tr_parts, 4 * (size_t)trace_steps);
+    }
+}
+
+int pow2_at_least(int v) {
+    int p = 64;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+void rebuild_chains(alma_ctx* ctx, const std::vector<int>& ids, int count) {
+    ChainBatch& b = ctx->chains;
+    HIP_TRY(hipMemcpyAsync(b.list, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_chain_rebuild(b, ctx->sys, (int)ids.size(), count, ctx->stream));
+    std::vector<int> fl(4 * (size_t)b.nch);
+    HIP_TRY(hipMemcpyAsync(fl.data(), b.flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int c : ids) {
+        if (fl[4 * c] & 2) throw Fail{ALMA_ERUNTIME, "chain expanded beyond its cell grid"};
+        if (fl[4 * c] & 5) throw Fail{ALMA_ERUNTIME, "neighbour rows exceeded their capacity"};
+    }
+}
+
+void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_stats* stats, double* snap_xyz,
+            double* snap_e) {
+    DevSys& s = ctx->sys;
+    if (s.n == 0) throw Fail{ALMA_EINVAL, "initial_state empty"};
+    if (!s.has_topo || s.topo.n_rot <= 0) throw Fail{ALMA_EINVAL, "topology has no rotatable bonds"};
+    if (p.n_chains <= 0 || p.steps <= 0) throw Fail{ALMA_EINVAL, "ncand/steps must be positive"};
+    if (p.mode < ALMA_MC_ENSEMBLE || p.mode > ALMA_MC_TORSION30)
+        throw Fail{ALMA_EINVAL, "unsupported MC mode"};
+    if (p.snapshot_every < 0) throw Fail{ALMA_EINVAL, "snapshot interval must be >= 0"};
+    if (!(p.temperature > 0.0)) throw Fail{ALMA_EINVAL, "temperature must be positive"};
+    const int nch = p.n_chains, n = s.n;
+    const int trace_steps = p.trace ? p.steps : 0;
+    // size rows from a full evaluation of the initial state
+    evaluate(ctx, nullptr);
+    int h_flags[2];
+    HIP_TRY(hipMemcpy(h_flags, s.flags, sizeof(h_flags), hipMemcpyDeviceToHost));
+    ChainBatch& b = ctx->chains;
+    alloc_chains(b, s, nch, trace_steps);
+    int maxh = std::max(h_flags[1], 32);
+    b.cap_h = (maxh * 3 / 2 + 63) / 64 * 64;
+    b.cap_f = pow2_at_least(3 * maxh);
+    b.cap_s = (std::max(64, maxh) + 63) / 64 * 64;
+    alloc_rows(b);
+    // initial state of every chain
+    {
+        std::vector<double> p4(4 * (size_t)n);
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) p4[4 * i + d] = ctx->h_pos[3 * i + d], p4[4 * i + 3] = 0.0;
+        for (int c = 0; c < nch; ++c)
+            HIP_TRY(hipMemcpy(b.pos + (size_t)c * 4 * n, p4.data(), sizeof(double) * p4.size(),
+                              hipMemcpyHostToDevice));
+    }
+    std::vector<ChainState> st(nch);
+    std::vector<HostMT> mt;
+    std::vector<std::vector<uint32_t>> wbuf(nch, std::vector<uint32_t>(b.wcap));
+    std::vector<long long> consumed(nch, 0);
+    mt.reserve(nch);
+    for (int c = 0; c < nch; ++c) {
+        mt.emplace_back(p.seed + (uint32_t)c);
+        for (auto& w : wbuf[c]) w = mt[c]();
+        HIP_TRY(hipMemcpy(b.words + (size_t)c * b.wcap, wbuf[c].data(), sizeof(uint32_t) * b.wcap,
+                          hipMemcpyHostToDevice));
+        ChainState& x = st[c];
+        x = ChainState{};
+        x.cur_max = p.max_angle;
+        x.temperature = p.temperature;
+        x.mala_step = p.mala_step;
+        x.wavail = b.wcap;
+        x.mode = p.mode;
+        x.status = CS_REBUILD;
+        x.steps_target = p.steps;
+    }
+    const int nsnap = p.snapshot_every > 0 ? p.steps / p.snapshot_every : 0;
+    std::vector<int> snaps_done(nch, 0);
+    auto target_of = [&](int c) {
+        if (nsnap && snaps_done[c] < nsnap) return std::min(p.steps, (snaps_done[c] + 1) * p.snapshot_every);
+        return p.steps;
+    };
+    for (int c = 0; c < nch; ++c) st[c].steps_target = target_of(c);
+    HIP_TRY(hipMemcpy(b.st, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice));
+    // initial lists (the first build is not a rebuild)
+    std::vector<int> all(nch);
+    for (int c = 0; c < nch; ++c) all[c] = c;
+    rebuild_chains(ctx, all, 0);
+
+    std::vector<double> snap_pos(4 * (size_t)n);
+    for (int round = 0;; ++round) {
+        HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
+        std::vector<int> rb, run;
+        bool dirty = false;
+        for (int c = 0; c < nch; ++c) {
+            ChainState& x = st[c];
+            if (x.status == CS_ERROR) throw Fail{ALMA_ERUNTIME, "MC chain ran out of random words"};
+            if (x.status == CS_WORDS) {
+                // keep the unconsumed tail, append fresh words
+                int tail = x.wavail - x.cursor;
+                consumed[c] += x.cursor;
+                std::vector<uint32_t> nb(b.wcap);
+                std::copy(wbuf[c].begin() + x.cursor, wbuf[c].begin() + x.wavail, nb.begin());
+                for (int k = tail; k < b.wcap; ++k) nb[k] = mt[c]();
+                wbuf[c].swap(nb);
+                HIP_TRY(hipMemcpy(b.words + (size_t)c * b.wcap, wbuf[c].data(), sizeof(uint32_t) * b.wcap,
+                                  hipMemcpyHostToDevice));
+                x.cursor = 0;
+                x.wavail = b.wcap;
+                x.status = CS_RUN;
+                dirty = true;
+            }
+            if (x.status == CS_TARGET && x.steps_done < p.steps + 1 && nsnap && snaps_done[c] < nsnap &&
+                x.steps_done == (snaps_done[c] + 1) * p.snapshot_every) {
+                // snapshot + fresh calculate_potential (src/physics_engine.cpp:2914-2920)
+                HIP_TRY(hipMemcpy(snap_pos.data(), b.pos + (size_t)c * 4 * n, sizeof(double) * 4 * n,
+                                  hipMemcpyDeviceToHost));
+                int k = snaps_done[c];
+                if (snap_xyz)
+                    for (int i = 0; i < n; ++i)
+                        for (int d = 0; d < 3; ++d)
+                            snap_xyz[((size_t)c * nsnap + k) * 3 * n + 3 * i + d] = snap_pos[4 * i + d];
+                if (snap_e) {
+                    HIP_TRY(hipMemcpy(s.pos, snap_pos.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+                    double terms[ALMA_NTERMS];
+                    evaluate(ctx, terms);
+                    snap_e[(size_t)c * nsnap + k] = terms[ALMA_TOTAL];
+                }
+                snaps_done[c] += 1;
+                x.steps_target = target_of(c);
+                if (x.steps_done < p.steps) x.status = CS_RUN;
+                dirty = true;
+            }
+            if (x.status == CS_TARGET && x.steps_done < x.steps_target) {
+                x.status = CS_RUN;
+                dirty = true;
+            }
+            if (x.status == CS_REBUILD) rb.push_back(c);
+            if (x.status == CS_RUN || x.status == CS_REBUILD) run.push_back(c);
+        }
+        if (run.empty()) break;
+        if (dirty) HIP_TRY(hipMemcpy(b.st, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice));
+        if (!rb.empty()) rebuild_chains(ctx, rb, 1);
+        HIP_TRY(hipMemcpyAsync(b.list, run.data(), sizeof(int) * run.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(launch_chain_mc(b, s, (int)run.size(), ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    // results
+    std::vector<double> p4(4 * (size_t)n);
+    for (int c = 0; c < nch; ++c) {
+        HIP_TRY(hipMemcpy(p4.data(), b.pos + (size_t)c * 4 * n, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) xyz_out[(size_t)c * 3 * n + 3 * i + d] = p4[4 * i + d];
+        stats[c].cur_e = st[c].cur_e;
+        stats[c].cur_max = st[c].cur_max;
+        stats[c].accepted = st[c].accepted;
+        stats[c].moves = st[c].moves;
+        stats[c].rebuilds = st[c].rebuilds;
+        stats[c].words = consumed[c] + st[c].cursor;
+    }
+    if (trace_steps) {
+        ctx->tr_kind.resize(trace_steps);
+        ctx->tr_bond.resize(trace_steps);
+        ctx->tr_delta.resize(trace_steps);
+        ctx->tr_dE.resize(trace_steps);
+        ctx->tr_acc.resize(trace_steps);
+        ctx->tr_parts.resize(4 * (size_t)trace_steps);
+        HIP_TRY(hipMemcpy(ctx->tr_parts.data(), b.tr_parts, sizeof(double) * 4 * trace_steps,
+                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ctx->tr_kind.data(), b.tr_kind, sizeof(int) * trace_steps, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ctx->tr_bond.data(), b.tr_bond, sizeof(int) * trace_steps, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ctx->tr_delta.data(), b.tr_delta, sizeof(double) * trace_steps, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ctx->tr_dE.data(), b.tr_dE, sizeof(double) * trace_steps, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ctx->tr_acc.data(), b.tr_acc, sizeof(int) * trace_steps, hipMemcpyDeviceToHost));
+    }
+    // device coordinates of the system back to the loaded state
+    upload_positions(ctx);
+    ctx->have_eval = false;
+}
+
 }  // namespace
 
 extern "C" {
@@ -288,6 +581,7 @@ void alma_destroy(alma_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     free_sys(ctx->sys);
+    free_chains(ctx->chains);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -312,6 +606,7 @@ alma_status alma_set_system(alma_ctx* ctx, int32_t n, const double* xyz, const d
             if (!(rstar[i] > 0.0)) throw Fail{ALMA_EINVAL, "particle radius must be positive"};
         HIP_TRY(hipSetDevice(ctx->device));
         free_sys(ctx->sys);
+        free_chains(ctx->chains);
         DevSys& s = ctx->sys;
         s.n = n;
         ctx->have_eval = false;
@@ -325,6 +620,7 @@ alma_status alma_set_system(alma_ctx* ctx, int32_t n, const double* xyz, const d
         for (int i = 0; i < n; ++i) prm[i] = make_float4((float)q[i], (float)rstar[i], (float)eps[i], 0.f);
         dev_upload(&s.prm, prm.data(), n);
         dev_upload(&s.rstar, rstar, n);
+        dev_upload(&s.eps64, eps, n);
         dev_alloc(&s.born_sum, n);
         dev_alloc(&s.half_sum, n);
         dev_alloc(&s.a, n);
@@ -411,24 +707,22 @@ alma_status alma_born_radii(alma_ctx* ctx, double* a) {
 
 alma_status alma_mc_run(alma_ctx* ctx, const alma_mc_params* p, double* xyz_out, alma_mc_stats* stats,
                         double* snap_xyz, double* snap_e) {
-    (void)p;
-    (void)xyz_out;
-    (void)stats;
-    (void)snap_xyz;
-    (void)snap_e;
-    return guarded(ctx, [&] { throw Fail{ALMA_ERUNTIME, "device MC is not built into this library"}; });
+    return guarded(ctx, [&] {
+        if (!p || !xyz_out || !stats) throw Fail{ALMA_EINVAL, "MC arguments missing"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        run_mc(ctx, *p, xyz_out, stats, snap_xyz, snap_e);
+    });
 }
 
 alma_status alma_mc_trace(alma_ctx* ctx, int32_t* kind, int32_t* bond, double* delta, double* dE,
-                          int32_t* accepted) {
+                          int32_t* accepted, double* dE_parts) {
     return guarded(ctx, [&] {
-        size_t m = ctx->tr_kind.size();
         if (kind) std::copy(ctx->tr_kind.begin(), ctx->tr_kind.end(), kind);
         if (bond) std::copy(ctx->tr_bond.begin(), ctx->tr_bond.end(), bond);
         if (delta) std::copy(ctx->tr_delta.begin(), ctx->tr_delta.end(), delta);
         if (dE) std::copy(ctx->tr_dE.begin(), ctx->tr_dE.end(), dE);
         if (accepted) std::copy(ctx->tr_acc.begin(), ctx->tr_acc.end(), accepted);
-        (void)m;
+        if (dE_parts) std::copy(ctx->tr_parts.begin(), ctx->tr_parts.end(), dE_parts);
     });
 }
 

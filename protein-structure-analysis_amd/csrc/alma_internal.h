@@ -75,6 +75,7 @@ struct DevSys {
     double* pos = nullptr;       // 4n: x, y, z, 0
     float4* prm = nullptr;       // q, R*, eps, a (Born radius)
     double* rstar = nullptr;     // FP64 R*
+    double* eps64 = nullptr;     // FP64 epsilon
     double* born_sum = nullptr;  // full HCT sum per atom
     double* half_sum = nullptr;  // half-list HCT sum per atom
     double* a = nullptr;         // FP64 Born radius
@@ -95,7 +96,71 @@ struct DevSys {
     bool has_topo = false;
 };
 
-// ---- kernels / launchers (kernels.hip) ----
+// ---- device MC (mc.hip) ----
+// Chain status codes written by the MC kernel.
+enum { CS_RUN = 0, CS_REBUILD = 1, CS_TARGET = 2, CS_WORDS = 3, CS_ERROR = 4 };
+
+// State of one MC chain (device resident).
+struct ChainState {
+    double cur_e;        // running energy (curE)
+    double cur_max;      // adaptive step
+    double temperature;  // kT, kcal/mol
+    double mala_step;
+    long long accepted, moves, rebuilds;
+    int cursor;          // next word in this chain's RNG buffer
+    int wavail;          // valid words in the buffer
+    int steps_done;
+    int steps_target;    // stop (and report) at this step count
+    int status;
+    int acc_win, tot_win;
+    int mode;
+};
+
+// Batch of chains of one system (same topology and parameters, own state).
+// Per-chain arrays are chain-major with the strides below.
+struct ChainBatch {
+    int n = 0, nch = 0;
+    int cap_h = 0, cap_f = 0, cap_s = 0, wcap = 0, max_cells = 0;
+    double* pos = nullptr;    // nch * 4n current coordinates
+    double* posn = nullptr;   // nch * 4n trial coordinates of moved atoms
+    double* ref = nullptr;    // nch * 4n coordinates at the last list build
+    double* a = nullptr;      // nch * n Born radii (reference semantics, may be stale)
+    double* anew = nullptr;   // nch * n trial radii of moved atoms
+    double* half = nullptr;   // nch * n half-list HCT sums at current coordinates
+    double* dhalf = nullptr;  // nch * n per-step change of half (zero between steps)
+    double* born_sum = nullptr;
+    double* part = nullptr;   // nch * 5n
+    double* rec = nullptr;    // nch * (n_dih + n_ss)
+    int* flag = nullptr;      // nch * n atoms whose SASA fold a move touches
+    uint32_t* nlh = nullptr;  // nch * n * cap_h half rows, cell-scan order, excl bit
+    uint32_t* nlf = nullptr;  // nch * n * cap_f full rows, ascending index, excl bit
+    uint32_t* nls = nullptr;  // nch * n * cap_s half rows within SASA reach + skin
+    int* nlh_cnt = nullptr;
+    int* nlf_cnt = nullptr;
+    int* nls_cnt = nullptr;
+    int* cell_of = nullptr;   // nch * n
+    int* cell_cnt = nullptr;  // nch * (max_cells + 1)
+    int* cell_start = nullptr;
+    int* cell_atoms = nullptr;
+    int* cell_tmp = nullptr;
+    Grid* grid = nullptr;     // nch
+    int* flags = nullptr;     // nch * 4: overflow bits, max half, max full, max sasa
+    uint32_t* words = nullptr;  // nch * wcap mt19937 output words
+    ChainState* st = nullptr;   // nch
+    int* list = nullptr;        // chain ids of a batched launch
+    int trace_cap = 0;
+    int* tr_kind = nullptr;
+    int* tr_bond = nullptr;
+    double* tr_delta = nullptr;
+    double* tr_dE = nullptr;
+    double* tr_parts = nullptr;   // 4 per step: pairs, SASA, dihedral, SS
+    int* tr_acc = nullptr;
+};
+
+hipError_t launch_chain_rebuild(ChainBatch& b, const DevSys& s, int nlist, int count, hipStream_t st);
+hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int nlist, hipStream_t st);
+
+// ---- kernels / launchers (energy.hip) ----
 hipError_t launch_build_nl(DevSys& s, hipStream_t st);
 hipError_t launch_born(DevSys& s, hipStream_t st);
 hipError_t launch_energy(DevSys& s, hipStream_t st);
@@ -109,10 +174,11 @@ struct alma_ctx {
     std::string err;
     std::string name;
     alma::DevSys sys;
+    alma::ChainBatch chains;
     std::vector<double> h_pos;       // host copy of the current coordinates
     std::vector<double> h_q, h_r, h_e;
     bool have_eval = false;
     // MC trace of the last run
     std::vector<int> tr_kind, tr_bond, tr_acc;
-    std::vector<double> tr_delta, tr_dE;
+    std::vector<double> tr_delta, tr_dE, tr_parts;
 };

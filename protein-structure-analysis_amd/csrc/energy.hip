@@ -6,7 +6,7 @@
 //   SASA half-list fold, dihedral and disulfide terms, final FP64 reduction.
 // One 64-lane wavefront owns one atom's neighbour row; rows are contiguous
 // (n x cap layout) so lanes read consecutive entries.
-#include "alma_internal.h"
+#include "physics_dev.h"
 
 #include <cmath>
 
@@ -14,30 +14,7 @@ namespace alma {
 
 namespace {
 
-constexpr int kWave = 64;
-constexpr int kBlock = 256;            // 4 wavefronts
-constexpr int kWavesPerBlock = kBlock / kWave;
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
-__device__ __forceinline__ float wave_sumf(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
-}
-
-// Reference FP64 squared distance, no contraction into FMA (the reference's
-// d^2 < 196 and d^2 > 144 decisions must be reproduced exactly).
-__device__ __forceinline__ double dist2_exact(const double* pi, const double* pj) {
-#pragma clang fp contract(off)
-    double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
-    return dx * dx + dy * dy + dz * dz;
-}
+using namespace dev;
 
 // ── neighbour list ──────────────────────────────────────────────────────────
 // NeighborList::build, src/physics_engine.cpp:293-342.
@@ -148,19 +125,6 @@ __global__ void k_cell_rank(int n, const int* __restrict__ cell_of, const int* _
     sorted[b + r] = i;
 }
 
-__device__ __forceinline__ bool in_sorted(const int* __restrict__ v, int b, int e, int key) {
-    while (b < e) {
-        int m = (b + e) >> 1;
-        int x = v[m];
-        if (x == key) return true;
-        if (x < key)
-            b = m + 1;
-        else
-            e = m;
-    }
-    return false;
-}
-
 // one wavefront per atom i: scan the 27 cells dz -> dy -> dx, cell members in
 // ascending order, keep j > i with d^2 < 196; ballot + prefix count keeps the
 // scan order when appending.
@@ -224,15 +188,6 @@ __global__ void __launch_bounds__(kBlock) k_nl_fill(const double* __restrict__ p
     }
 }
 
-// ── HCT Born radii ──────────────────────────────────────────────────────────
-// hct, src/physics_engine.cpp:2007-2011 (as written, FP32);
-// born_radii :2024-2039 sums both directions over every neighbour entry.
-__device__ __forceinline__ float hct_f(float r, float r2, float ri, float rj) {
-    float L = fmaxf(fabsf(r - rj), ri), U = r + rj;
-    if (ri >= U) return 0.0f;
-    return 1.0f / L - 1.0f / U + (r2 - rj * rj + ri * ri) / (2.0f * r * ri * ri) * logf(L / U) * 0.5f / r;
-}
-
 __global__ void __launch_bounds__(kBlock) k_born_sum(const double* __restrict__ pos,
                                                     const float4* __restrict__ prm, int n,
                                                     const uint32_t* __restrict__ nl,
@@ -272,39 +227,10 @@ __global__ void k_born_finalize(int n, const double* __restrict__ rstar,
     prm[i].w = (float)v;
 }
 
-// ── fused pair energy ───────────────────────────────────────────────────────
-// pair_e / pair_e_diag, src/physics_engine.cpp:2148-2199.  Returns the four
-// terms DH, GB, LJ, hard-core in FP32.
-struct PairTerms {
-    float dh, gb, lj, hc;
-};
-
-__device__ __forceinline__ PairTerms pair_terms_f(float r2, float4 pi, float4 pj) {
-    PairTerms t;
-    float r = sqrtf(r2);
-    float qp = pi.x * pj.x;
-    float sig = pi.y + pj.y;
-    t.dh = ((float)(kCoulomb / kEpsWater) * qp / r) * expf(-(float)kKappa * r);
-    float D = pi.w * pj.w;
-    float fgb = sqrtf(r2 + D * expf(-r2 / (4.0f * D)));
-    t.gb = (float)kGbCoef * qp / fgb;
-    float eps = sqrtf(pi.z * pj.z);
-    float s2 = sig * sig / r2;
-    float s6 = s2 * s2 * s2;
-    t.lj = 4.0f * eps * (s6 * s6 - s6);
-    t.hc = 0.0f;
-    float rc = sig * (float)kHardFrac;
-    if (r < rc) {
-        float c2 = rc * rc / r2;
-        float c6 = c2 * c2 * c2;
-        float x = c6 * c6 - 1.0f;
-        t.hc = fminf((float)kHardScale * x * x, (float)kHardCap);
-    }
-    return t;
-}
-
 __global__ void __launch_bounds__(kBlock) k_pair_energy(const double* __restrict__ pos,
-                                                       const float4* __restrict__ prm, int n,
+                                                       const float4* __restrict__ prm,
+                                                       const double* __restrict__ rstar,
+                                                       const double* __restrict__ eps64, int n,
                                                        const uint32_t* __restrict__ nl,
                                                        const int* __restrict__ cnt, int cap,
                                                        int use_excl, double* __restrict__ part) {
@@ -322,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_pair_energy(const double* __restrict
         int j = (int)(e & kIdxMask);
         double r2d = dist2_exact(pi, pos + 4 * j);
         if (r2d > kPairCut2) continue;
-        PairTerms t = pair_terms_f((float)r2d, qi, prm[j]);
+        PairTerms t = pair_terms(r2d, qi, prm[j], rstar[i], rstar[j], eps64[i], eps64[j]);
         dh += t.dh;
         gb += t.gb;
         lj += t.lj;
@@ -365,25 +291,6 @@ __global__ void k_sasa(const double* __restrict__ pos, const float4* __restrict_
         sa -= fmin(sa * 0.85, k2 * (dc - r));
     }
     part[5 * i + 4] = kGammaSA * fmax(0.0, sa);
-}
-
-// ── dihedral / disulfide (FP64) ─────────────────────────────────────────────
-// dihedral_angle :1871-1885 (atan2 of the reference's non-IUPAC pair),
-// dihedral_e :1912-1921, ss_e :1965-1974.
-__device__ double dih_angle_d(const double* pos, int a, int b, int c, int d) {
-    double b1[3], b2[3], b3[3];
-    for (int k = 0; k < 3; ++k) {
-        b1[k] = pos[4 * a + k] - pos[4 * b + k];
-        b2[k] = pos[4 * c + k] - pos[4 * b + k];
-        b3[k] = pos[4 * d + k] - pos[4 * c + k];
-    }
-    double n1x = b1[1] * b2[2] - b1[2] * b2[1], n1y = b1[2] * b2[0] - b1[0] * b2[2],
-           n1z = b1[0] * b2[1] - b1[1] * b2[0];
-    double n2x = b2[1] * b3[2] - b2[2] * b3[1], n2y = b2[2] * b3[0] - b2[0] * b3[2],
-           n2z = b2[0] * b3[1] - b2[1] * b3[0];
-    double m1x = n1y * b2[2] - n1z * b2[1], m1y = n1z * b2[0] - n1x * b2[2],
-           m1z = n1x * b2[1] - n1y * b2[0];
-    return atan2(m1x * n2x + m1y * n2y + m1z * n2z, n1x * n2x + n1y * n2y + n1z * n2z);
 }
 
 __global__ void k_bonded(const double* __restrict__ pos, DevTopo t, double* __restrict__ rec) {
@@ -440,8 +347,8 @@ inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 
 hipError_t launch_build_nl(DevSys& s, hipStream_t st) {
     const int n = s.n;
-    hipMemsetAsync(s.flags, 0, 2 * sizeof(int), st);
-    hipMemsetAsync(s.cell_cnt, 0, sizeof(int) * (s.max_cells + 1), st);
+    (void)hipMemsetAsync(s.flags, 0, 2 * sizeof(int), st);
+    (void)hipMemsetAsync(s.cell_cnt, 0, sizeof(int) * (s.max_cells + 1), st);
     k_bbox<<<1, kBlock, 0, st>>>(s.pos, n, s.grid);
     k_cell_assign<<<blocks_for(n, kBlock), kBlock, 0, st>>>(s.pos, n, s.grid, s.max_cells, s.cell_of,
                                                             s.cell_cnt, s.flags);
@@ -459,7 +366,7 @@ hipError_t launch_build_nl(DevSys& s, hipStream_t st) {
 
 hipError_t launch_born(DevSys& s, hipStream_t st) {
     const int n = s.n;
-    hipMemsetAsync(s.born_sum, 0, sizeof(double) * n, st);
+    (void)hipMemsetAsync(s.born_sum, 0, sizeof(double) * n, st);
     k_born_sum<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s.pos, s.prm, n, s.nl, s.nl_cnt,
                                                                  s.cap, s.born_sum, s.half_sum);
     k_born_finalize<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.rstar, s.born_sum, s.a, s.prm);
@@ -468,7 +375,7 @@ hipError_t launch_born(DevSys& s, hipStream_t st) {
 
 hipError_t launch_energy(DevSys& s, hipStream_t st) {
     const int n = s.n;
-    k_pair_energy<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s.pos, s.prm, n, s.nl, s.nl_cnt,
+    k_pair_energy<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s.pos, s.prm, s.rstar, s.eps64, n, s.nl, s.nl_cnt,
                                                                     s.cap, s.has_topo ? 1 : 0, s.part);
     k_sasa<<<blocks_for(n, kBlock), kBlock, 0, st>>>(s.pos, s.prm, n, s.nl, s.nl_cnt, s.cap, s.part);
     int nrec = s.has_topo ? s.topo.n_dih + s.topo.n_ss : 0;

@@ -1,0 +1,1066 @@
+// mc.hip -- device-resident torsion Metropolis MC on gfx950.
+//
+// One 1024-thread workgroup runs one chain for many steps without a host
+// round trip (the reference pays two host<->device copies and a sync per dE,
+// src/physics_engine_cuda.cu:841-858).  A step follows try_torsion /
+// try_crankshaft / try_concerted_sidechain (src/physics_engine.cpp:2328-2501,
+// :2782-2878): the same RNG draws in the same order, dE = cross-pair change +
+// SASA change + boundary dihedrals + side disulfides, Born radii of moved
+// atoms from the half neighbour list (update_born, :2043-2051).  What changes
+// is the work per step:
+//   * cross pairs are found from j-sorted full neighbour rows by binary
+//     search against the moved set's index runs, from the smaller of the
+//     moved set and its complement (no pass over all pairs);
+//   * update_born uses a maintained per-atom half-list HCT sum plus the
+//     change over cross pairs only (pairs inside the moved set keep their
+//     distance under a rigid rotation);
+//   * SASA is re-folded only for atoms whose half row holds a cross partner,
+//     over a sub-row kept in neighbour-list order.
+// The list is rebuilt (batched over chains) by the host between launches
+// whenever an accepted move drifts an atom more than 1 A
+// (needs_rebuild, :345-351), with a fresh full energy as the reference does.
+#include "physics_dev.h"
+
+namespace alma {
+
+namespace {
+
+using namespace dev;
+
+constexpr int kMcBlock = 1024;
+constexpr int kMcWaves = kMcBlock / kWave;
+constexpr int kSortBlock = 64;
+constexpr double kSasaSkin = 4.0;    // A beyond contact kept in the SASA sub-row
+constexpr double kSasaTrial = 1.5;   // max trial displacement served by it
+
+__device__ __forceinline__ double* P4(const ChainBatch& b, double* base, int c) {
+    return base + (size_t)c * 4 * b.n;
+}
+__device__ __forceinline__ size_t R1(const ChainBatch& b, int c) { return (size_t)c * b.n; }
+
+// ───────────────────────── batched list rebuild ─────────────────────────────
+
+__global__ void rb_clear(ChainBatch b) {
+    int c = b.list[blockIdx.y];
+    size_t o = R1(b, c);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
+        b.nlf_cnt[o + i] = 0;
+        b.born_sum[o + i] = 0.0;
+    }
+    size_t cb = (size_t)c * (b.max_cells + 1);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= b.max_cells; k += gridDim.x * blockDim.x)
+        b.cell_cnt[cb + k] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 4) b.flags[4 * c + threadIdx.x] = 0;
+}
+
+__global__ void rb_bbox(ChainBatch b, double cs) {
+    int c = b.list[blockIdx.y];
+    const double* pos = b.pos + (size_t)c * 4 * b.n;
+    Grid* g = b.grid + c;
+    __shared__ double slo[3][kBlock], shi[3][kBlock];
+    int t = threadIdx.x;
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = pos[d];
+    for (int i = t; i < b.n; i += blockDim.x)
+        for (int d = 0; d < 3; ++d) {
+            double v = pos[4 * i + d];
+            lo[d] = fmin(lo[d], v);
+            hi[d] = fmax(hi[d], v);
+        }
+    for (int d = 0; d < 3; ++d) {
+        slo[d][t] = lo[d];
+        shi[d][t] = hi[d];
+    }
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int d = 0; d < 3; ++d) {
+                slo[d][t] = fmin(slo[d][t], slo[d][t + s]);
+                shi[d][t] = fmax(shi[d][t], shi[d][t + s]);
+            }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma clang fp contract(off)
+        int ncell = 1;
+        g->cs = cs;
+        for (int d = 0; d < 3; ++d) {
+            double l = slo[d][0] - cs;
+            double len = shi[d][0] - l + 2.0 * cs;
+            int k = (int)ceil(len / cs);
+            g->lo[d] = l;
+            g->nc[d] = k < 1 ? 1 : k;
+            ncell *= g->nc[d];
+        }
+        g->ncell = ncell;
+    }
+}
+
+__global__ void rb_cell_assign(ChainBatch b) {
+    int c = b.list[blockIdx.y];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    const double* pos = b.pos + (size_t)c * 4 * b.n;
+    const Grid* g = b.grid + c;
+    int cc[3];
+    for (int d = 0; d < 3; ++d) {
+        int v = (int)floor((pos[4 * i + d] - g->lo[d]) / g->cs);
+        cc[d] = min(g->nc[d] - 1, max(0, v));
+    }
+    int id = cc[0] + g->nc[0] * (cc[1] + g->nc[1] * cc[2]);
+    if (g->ncell > b.max_cells) {
+        atomicOr(&b.flags[4 * c], 2);
+        id = 0;
+    }
+    b.cell_of[R1(b, c) + i] = id;
+    atomicAdd(&b.cell_cnt[(size_t)c * (b.max_cells + 1) + id], 1);
+}
+
+__global__ void rb_cell_scan(ChainBatch b) {
+    int c = b.list[blockIdx.y];
+    int* cnt = b.cell_cnt + (size_t)c * (b.max_cells + 1);
+    int* start = b.cell_start + (size_t)c * (b.max_cells + 1);
+    __shared__ int buf[1024];
+    int ncell = min(b.grid[c].ncell, b.max_cells);
+    int t = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < ncell; base += 1024) {
+        int v = (base + t < ncell) ? cnt[base + t] : 0;
+        buf[t] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int add = t >= o ? buf[t - o] : 0;
+            __syncthreads();
+            buf[t] += add;
+            __syncthreads();
+        }
+        if (base + t < ncell) start[base + t] = carry + buf[t] - v;
+        int tot = buf[1023];
+        __syncthreads();
+        carry += tot;
+    }
+    if (t == 0) start[ncell] = carry;
+    for (int k = t; k < ncell; k += 1024) cnt[k] = 0;
+}
+
+__global__ void rb_cell_place(ChainBatch b) {
+    int c = b.list[blockIdx.y];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    int cl = b.cell_of[R1(b, c) + i];
+    size_t cb = (size_t)c * (b.max_cells + 1);
+    b.cell_tmp[R1(b, c) + b.cell_start[cb + cl] + atomicAdd(&b.cell_cnt[cb + cl], 1)] = i;
+}
+
+__global__ void rb_cell_rank(ChainBatch b) {
+    int c = b.list[blockIdx.y];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    int cl = b.cell_of[R1(b, c) + i];
+    size_t cb = (size_t)c * (b.max_cells + 1);
+    int s0 = b.cell_start[cb + cl], s1 = b.cell_start[cb + cl + 1];
+    const int* tmp = b.cell_tmp + R1(b, c);
+    int r = 0;
+    for (int k = s0; k < s1; ++k) r += tmp[k] < i;
+    b.cell_atoms[R1(b, c) + s0 + r] = i;
+}
+
+// Half rows in the reference cell-scan order (NeighborList::build), the SASA
+// sub-row of entries closer than R*_i + R*_j + 2*probe + kSasaSkin at build
+// time, and both directions of every pair appended to the full rows (sorted
+// later).  Accepted coordinates stay within 1 A of the build positions; a
+// trial move may carry atoms further, so the MC step folds over the sub-row
+// only while every moved atom is within kSasaTrial of its build position
+// (then no pair can close by more than kSasaSkin) and over the whole half row
+// otherwise.
+__global__ void __launch_bounds__(kBlock) rb_nl_fill(ChainBatch b, DevSys s) {
+    int c = b.list[blockIdx.y];
+    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    int* fl = b.flags + 4 * c;
+    if (i >= b.n || (fl[0] & 2)) return;
+    int lane = lane_id();
+    const double* pos = b.pos + (size_t)c * 4 * b.n;
+    const Grid* g = b.grid + c;
+    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+    const double ri = s.rstar[i];
+    int cl = b.cell_of[R1(b, c) + i];
+    int cx = cl % g->nc[0], cy = (cl / g->nc[0]) % g->nc[1], cz = cl / (g->nc[0] * g->nc[1]);
+    const int* start = b.cell_start + (size_t)c * (b.max_cells + 1);
+    const int* atoms = b.cell_atoms + R1(b, c);
+    int eb = 0, ee = 0;
+    const int* exo = s.has_topo ? s.topo.excl_off : nullptr;
+    if (exo) {
+        eb = exo[i];
+        ee = exo[i + 1];
+    }
+    uint32_t* rowh = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+    uint32_t* rows = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
+    int* cntf = b.nlf_cnt + R1(b, c);
+    uint32_t* nlf = b.nlf + R1(b, c) * (size_t)b.cap_f;
+    int nh = 0, ns = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        if (z < 0 || z >= g->nc[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            if (y < 0 || y >= g->nc[1]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                int x = cx + dx;
+                if (x < 0 || x >= g->nc[0]) continue;
+                int cc = x + g->nc[0] * (y + g->nc[1] * z);
+                int e = start[cc + 1];
+                for (int base = start[cc]; base < e; base += kWave) {
+                    int k = base + lane;
+                    bool keep = false, near = false;
+                    int j = 0;
+                    double d2 = 0.0;
+                    if (k < e) {
+                        j = atoms[k];
+                        if (j > i) {
+                            d2 = dist2_exact(pi, pos + 4 * j);
+                            keep = d2 < kNlRcut2;
+                            double reach = ri + s.rstar[j] + 2.0 * kProbe + kSasaSkin;
+                            near = keep && d2 < reach * reach;
+                        }
+                    }
+                    uint64_t m = __ballot(keep);
+                    uint64_t mn = __ballot(near);
+                    uint64_t below = (1ull << lane) - 1ull;
+                    if (keep) {
+                        uint32_t v = (uint32_t)j;
+                        if (exo && in_sorted(s.topo.excl_idx, eb, ee, j)) v |= kExclBit;
+                        int slot = nh + __popcll(m & below);
+                        if (slot < b.cap_h) rowh[slot] = v;
+                        if (near) {
+                            int ss = ns + __popcll(mn & below);
+                            if (ss < b.cap_s) rows[ss] = v;
+                        }
+                        int fj = atomicAdd(&cntf[j], 1);
+                        if (fj < b.cap_f) nlf[(size_t)j * b.cap_f + fj] = (uint32_t)i | (v & kExclBit);
+                        int fi = atomicAdd(&cntf[i], 1);
+                        if (fi < b.cap_f) nlf[(size_t)i * b.cap_f + fi] = v;
+                    }
+                    nh += __popcll(m);
+                    ns += __popcll(mn);
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        b.nlh_cnt[R1(b, c) + i] = nh;
+        b.nls_cnt[R1(b, c) + i] = ns;
+        atomicMax(&fl[1], nh);
+        atomicMax(&fl[3], ns);
+        if (nh > b.cap_h || ns > b.cap_s) atomicOr(&fl[0], 1);
+    }
+}
+
+// ascending sort of one full row (bitonic, one wavefront per row)
+__global__ void __launch_bounds__(kSortBlock) rb_full_sort(ChainBatch b) {
+    extern __shared__ uint32_t sh[];
+    int c = b.list[blockIdx.y];
+    int i = blockIdx.x;
+    int cnt = b.nlf_cnt[R1(b, c) + i];
+    int* fl = b.flags + 4 * c;
+    if (threadIdx.x == 0) {
+        atomicMax(&fl[2], cnt);
+        if (cnt > b.cap_f) atomicOr(&fl[0], 4);
+    }
+    cnt = min(cnt, b.cap_f);
+    uint32_t* row = b.nlf + (R1(b, c) + i) * (size_t)b.cap_f;
+    int P = 1;
+    while (P < cnt) P <<= 1;
+    for (int k = threadIdx.x; k < P; k += kSortBlock) sh[k] = k < cnt ? row[k] : 0xffffffffu;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < P; t += kSortBlock) {
+                int u = t ^ j;
+                if (u > t) {
+                    uint32_t a = sh[t], bb = sh[u];
+                    uint32_t ka = a & kIdxMask, kb = bb & kIdxMask;
+                    if (a == 0xffffffffu) ka = 0xffffffffu;
+                    if (bb == 0xffffffffu) kb = 0xffffffffu;
+                    bool up = (t & k) == 0;
+                    if ((ka > kb) == up) {
+                        sh[t] = bb;
+                        sh[u] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < cnt; k += kSortBlock) row[k] = sh[k];
+}
+
+__global__ void __launch_bounds__(kBlock) rb_born_sum(ChainBatch b, DevSys s) {
+    int c = b.list[blockIdx.y];
+    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (i >= b.n) return;
+    int lane = lane_id();
+    const double* pos = b.pos + (size_t)c * 4 * b.n;
+    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+    float ri = s.prm[i].y;
+    int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
+    const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+    double* sum = b.born_sum + R1(b, c);
+    double acc = 0.0;
+    for (int k = lane; k < m; k += kWave) {
+        int j = (int)(row[k] & kIdxMask);
+        float r2 = (float)dist2_exact(pi, pos + 4 * j), r = sqrtf(r2);
+        float rj = s.prm[j].y;
+        acc += (double)hct_f(r, r2, ri, rj);
+        atomicAdd(&sum[j], (double)hct_f(r, r2, rj, ri));
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) {
+        atomicAdd(&sum[i], acc);
+        b.half[R1(b, c) + i] = acc;
+    }
+}
+
+__global__ void rb_born_fin(ChainBatch b, DevSys s) {
+    int c = b.list[blockIdx.y];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    size_t o = R1(b, c) + i;
+    b.a[o] = 1.0 / fmax(1.0 / s.rstar[i] - 0.5 * b.born_sum[o], 2.0);
+    for (int d = 0; d < 4; ++d) b.ref[4 * o + d] = b.pos[4 * o + d];
+    b.dhalf[o] = 0.0;
+    b.flag[o] = 0;
+}
+
+__device__ __forceinline__ float4 atom4(const DevSys& s, const double* a, int i) {
+    float4 v = s.prm[i];
+    v.w = (float)a[i];
+    return v;
+}
+
+__global__ void __launch_bounds__(kBlock) rb_pair(ChainBatch b, DevSys s) {
+    int c = b.list[blockIdx.y];
+    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (i >= b.n) return;
+    int lane = lane_id();
+    const double* pos = b.pos + (size_t)c * 4 * b.n;
+    const double* a = b.a + R1(b, c);
+    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+    float4 qi = atom4(s, a, i);
+    int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
+    const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+    double acc = 0.0;
+    for (int k = lane; k < m; k += kWave) {
+        uint32_t e = row[k];
+        if (e & kExclBit) continue;
+        int j = (int)(e & kIdxMask);
+        double r2 = dist2_exact(pi, pos + 4 * j);
+        if (r2 > kPairCut2) continue;
+        acc += pair_energy(r2, qi, atom4(s, a, j), s.rstar[i], s.rstar[j], s.eps64[i], s.eps64[j]);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) b.part[5 * (R1(b, c) + i)] = acc;
+}
+
+// SASA fold of atom i over its sub-row; `moved`/`posn` select trial
+// coordinates for atoms of the moved set (used by the MC step as well)
+__device__ double sasa_fold(const ChainBatch& b, const DevSys& s, int c, int i, const double* pi,
+                            const double* pos) {
+    double ri = (double)s.prm[i].y + kProbe;
+    double sa = 4.0 * M_PI * ri * ri;
+    const double k2 = M_PI * ri;
+    int m = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
+    const uint32_t* row = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
+    for (int k = 0; k < m; ++k) {
+        int j = (int)(row[k] & kIdxMask);
+        double dc = ri + (double)s.prm[j].y + kProbe;
+        double r2 = dist2_exact(pi, pos + 4 * j);
+        if (r2 >= dc * dc) continue;
+        double r = (double)sqrtf((float)r2);
+        if (r >= dc) continue;
+        sa -= fmin(sa * 0.85, k2 * (dc - r));
+    }
+    return kGammaSA * fmax(0.0, sa);
+}
+
+__global__ void rb_sasa(ChainBatch b, DevSys s) {
+    int c = b.list[blockIdx.y];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    const double* pos = b.pos + (size_t)c * 4 * b.n;
+    b.part[5 * (R1(b, c) + i) + 4] = sasa_fold(b, s, c, i, pos + 4 * i, pos);
+}
+
+__global__ void rb_total(ChainBatch b, DevSys s, int count_rebuild) {
+    int c = b.list[blockIdx.y];
+    __shared__ double buf[kBlock];
+    int t = threadIdx.x;
+    const double* pos = b.pos + (size_t)c * 4 * b.n;
+    double acc = 0.0;
+    for (int i = t; i < b.n; i += kBlock) acc += b.part[5 * (R1(b, c) + i)] + b.part[5 * (R1(b, c) + i) + 4];
+    if (s.has_topo) {
+        for (int k = t; k < s.topo.n_dih; k += kBlock) {
+            double phi = dih_angle_d(pos, s.topo.dih_a[k], s.topo.dih_b[k], s.topo.dih_c[k], s.topo.dih_d[k]);
+            for (int m = s.topo.dih_toff[k]; m < s.topo.dih_toff[k + 1]; ++m)
+                acc += s.topo.dih_v2[m] * (1.0 + cos((double)s.topo.dih_n[m] * phi - s.topo.dih_gamma[m]));
+        }
+        for (int k = t; k < s.topo.n_ss; k += kBlock) {
+            int i = s.topo.ss_i[k], j = s.topo.ss_j[k];
+            double dr = sqrt(dist2_exact(pos + 4 * i, pos + 4 * j)) - kR0ss;
+            acc += kKss * dr * dr;
+        }
+    }
+    buf[t] = acc;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (t < h) buf[t] += buf[t + h];
+        __syncthreads();
+    }
+    if (t == 0) {
+        b.st[c].cur_e = kBetaSA + buf[0];
+        if (count_rebuild) b.st[c].rebuilds += 1;
+        if (b.st[c].status == CS_REBUILD) b.st[c].status = CS_RUN;
+    }
+}
+
+// ───────────────────────────────── MC step ──────────────────────────────────
+
+// mt19937 words consumed with libstdc++-11 distribution semantics
+struct Words {
+    const uint32_t* w;
+    int cur, avail;
+    bool bad;
+    __device__ uint32_t next() {
+        if (cur >= avail) {
+            bad = true;
+            return 0u;
+        }
+        return w[cur++];
+    }
+    __device__ int uniform_int(int hi) {   // uniform_int_distribution<int>(0, hi)
+        uint32_t range = (uint32_t)hi + 1u;
+        uint64_t prod = (uint64_t)next() * (uint64_t)range;
+        uint32_t low = (uint32_t)prod;
+        if (low < range) {
+            uint32_t thr = (uint32_t)(-range) % range;
+            while (low < thr && !bad) {
+                prod = (uint64_t)next() * (uint64_t)range;
+                low = (uint32_t)prod;
+            }
+        }
+        return (int)(prod >> 32);
+    }
+    __device__ double canonical() {        // generate_canonical<double, 53>
+        double sum = (double)next();
+        sum += (double)next() * 4294967296.0;
+        double r = sum / 18446744073709551616.0;
+        return r >= 1.0 ? 0x1.fffffffffffffp-1 : r;
+    }
+    __device__ double uniform_real(double a, double bb) { return canonical() * (bb - a) + a; }
+};
+
+struct Runs {
+    const int2* r;
+    int n;
+    __device__ bool has(int i) const {
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            int m = (lo + hi) >> 1;
+            if (r[m].y < i)
+                lo = m + 1;
+            else
+                hi = m;
+        }
+        return lo < n && r[lo].x <= i;
+    }
+    __device__ int count() const {
+        int t = 0;
+        for (int k = 0; k < n; ++k) t += r[k].y - r[k].x + 1;
+        return t;
+    }
+    // k-th atom of the set
+    __device__ int at(int k) const {
+        for (int q = 0; q < n; ++q) {
+            int len = r[q].y - r[q].x + 1;
+            if (k < len) return r[q].x + k;
+            k -= len;
+        }
+        return -1;
+    }
+    // k-th atom of the complement in [0, N)
+    __device__ int gap_at(int k) const {
+        int prev = 0;
+        for (int q = 0; q < n; ++q) {
+            int len = r[q].x - prev;
+            if (k < len) return prev + k;
+            k -= len;
+            prev = r[q].y + 1;
+        }
+        return prev + k;
+    }
+};
+
+struct Move {
+    int valid, kind, bond, bond2, mode;
+    int nT;
+    double o[3], u[3], cs, sn;
+    double o2[3], u2[3], cs2, sn2;
+    double delta, dE;
+    double part[4];
+    int accept, stop, rebuild, words_at;
+};
+
+__device__ __forceinline__ void rodrigues_d(const double* p, const double* o, const double* u, double c,
+                                            double sn, double* out) {
+    double vx = p[0] - o[0], vy = p[1] - o[1], vz = p[2] - o[2];
+    double dot = u[0] * vx + u[1] * vy + u[2] * vz;
+    double cx = u[1] * vz - u[2] * vy, cy = u[2] * vx - u[0] * vz, cz = u[0] * vy - u[1] * vx;
+    double k = 1.0 - c;
+    out[0] = o[0] + vx * c + cx * sn + u[0] * dot * k;
+    out[1] = o[1] + vy * c + cy * sn + u[1] * dot * k;
+    out[2] = o[2] + vz * c + cz * sn + u[2] * dot * k;
+}
+
+__device__ bool axis_of(const double* pos, const DevTopo& t, int bond, double* o, double* u) {
+    int bi = t.rb_i[bond], bj = t.rb_j[bond];
+    for (int d = 0; d < 3; ++d) {
+        o[d] = pos[4 * bi + d];
+        u[d] = pos[4 * bj + d] - pos[4 * bi + d];
+    }
+    double l = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    if (l < 1e-10) return false;
+    u[0] /= l;
+    u[1] /= l;
+    u[2] /= l;
+    return true;
+}
+
+__device__ __forceinline__ int lower_bound_row(const uint32_t* row, int lo, int hi, int key) {
+    while (lo < hi) {
+        int m = (lo + hi) >> 1;
+        if ((int)(row[m] & kIdxMask) < key)
+            lo = m + 1;
+        else
+            hi = m;
+    }
+    return lo;
+}
+
+// One workgroup = one chain; runs until its step target, a needed list
+// rebuild, or the end of its RNG words.
+__global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
+    const int c = b.list[blockIdx.x];
+    const int n = b.n;
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+    const DevTopo& T = s.topo;
+    ChainState* cs = b.st + c;
+    double* pos = b.pos + (size_t)c * 4 * n;
+    double* posn = b.posn + (size_t)c * 4 * n;
+    const double* ref = b.ref + (size_t)c * 4 * n;
+    double* a = b.a + R1(b, c);
+    double* anew = b.anew + R1(b, c);
+    double* half = b.half + R1(b, c);
+    double* dhalf = b.dhalf + R1(b, c);
+    int* flag = b.flag + R1(b, c);
+    const uint32_t* nlf = b.nlf + R1(b, c) * (size_t)b.cap_f;
+    const int* nlf_cnt = b.nlf_cnt + R1(b, c);
+    const bool trace = b.trace_cap > 0 && c == 0;
+
+    __shared__ Move mv;
+    __shared__ double red[4 * kMcWaves];
+    __shared__ int far_move;
+    __shared__ Words W;
+
+    if (tid == 0) {
+        W.w = b.words + (size_t)c * b.wcap;
+        W.cur = cs->cursor;
+        W.avail = cs->wavail;
+        W.bad = false;
+    }
+    __syncthreads();
+
+    for (;;) {
+        // ---- move selection (lane 0 of wave 0; RNG order of the reference)
+        if (tid == 0) {
+            mv.stop = 0;
+            mv.valid = 0;
+            mv.accept = 0;
+            mv.rebuild = 0;
+            mv.dE = 0.0;
+            mv.delta = 0.0;
+            mv.part[0] = mv.part[1] = mv.part[2] = mv.part[3] = 0.0;
+            mv.kind = -1;
+            mv.bond = mv.bond2 = -1;
+            if (cs->steps_done >= cs->steps_target) {
+                cs->status = CS_TARGET;
+                mv.stop = 1;
+            } else if (W.avail - W.cur < 64) {
+                cs->status = CS_WORDS;
+                mv.stop = 1;
+            }
+            if (!mv.stop) {
+                mv.words_at = W.cur;
+                int mode = cs->mode;
+                int kind = 0;
+                if (mode == ALMA_MC_LANDSCAPE) {
+                    bool conc = T.n_csp > 0 && W.canonical() < 0.15;
+                    bool crank = !conc && T.n_cp > 0 && W.canonical() < 0.25;
+                    kind = conc ? 2 : crank ? 1 : 0;
+                } else if (mode == ALMA_MC_ENSEMBLE) {
+                    bool crank = T.n_cp > 0 && W.canonical() < 0.25;
+                    kind = crank ? 1 : 0;
+                }
+                mv.kind = kind;
+                if (kind == 0) {
+                    int k = W.uniform_int(T.n_rot - 1);
+                    mv.bond = k;
+                    if (T.side_off[k] != T.side_off[k + 1] && axis_of(pos, T, k, mv.o, mv.u)) {
+                        double d;
+                        if (mode == ALMA_MC_TORSION30) {
+                            d = W.uniform_real(-M_PI / 6.0, M_PI / 6.0);
+                        } else {
+                            double base = (T.rb_kind[k] == 0 || T.rb_kind[k] == 1) ? cs->cur_max
+                                                                                   : cs->cur_max * 2.5;
+                            base *= T.rb_scale[k];
+                            d = W.uniform_real(-base, base);
+                        }
+                        mv.delta = d;
+                        mv.cs = cos(d);
+                        mv.sn = sin(d);
+                        mv.valid = 1;
+                    }
+                } else if (kind == 1) {
+                    int cp = T.n_cp > 1 ? W.uniform_int(T.n_cp - 1) : 0;
+                    mv.bond = T.cp_phi[cp];
+                    mv.bond2 = T.cp_psi[cp];
+                    bool ok = T.side_off[mv.bond] != T.side_off[mv.bond + 1] &&
+                              T.side_off[mv.bond2] != T.side_off[mv.bond2 + 1];
+                    if (ok && axis_of(pos, T, mv.bond, mv.o, mv.u)) {
+                        double d = W.uniform_real(-cs->cur_max, cs->cur_max);
+                        mv.delta = d;
+                        mv.cs = cos(d);
+                        mv.sn = sin(d);
+                        mv.valid = 1;   // second axis checked after the first rotation
+                    }
+                } else {
+                    int pi = T.n_csp > 1 ? W.uniform_int(T.n_csp - 1) : 0;
+                    mv.bond = T.csp_a[pi];
+                    mv.bond2 = T.csp_b[pi];
+                    bool ok = T.side_off[mv.bond] != T.side_off[mv.bond + 1] &&
+                              T.side_off[mv.bond2] != T.side_off[mv.bond2 + 1];
+                    if (ok && axis_of(pos, T, mv.bond, mv.o, mv.u) &&
+                        axis_of(pos, T, mv.bond2, mv.o2, mv.u2)) {
+                        double dA = cs->cur_max * 2.5 * T.rb_scale[mv.bond];
+                        double dB = cs->cur_max * 2.5 * T.rb_scale[mv.bond2];
+                        double da = W.uniform_real(-dA, dA);
+                        double db = W.uniform_real(-dB, dB);
+                        mv.delta = da;
+                        mv.cs = cos(da);
+                        mv.sn = sin(da);
+                        mv.cs2 = cos(db);
+                        mv.sn2 = sin(db);
+                        mv.valid = 1;
+                    }
+                }
+                if (W.bad) {
+                    cs->status = CS_ERROR;
+                    mv.stop = 1;
+                }
+            }
+        }
+        __syncthreads();
+        if (mv.stop) break;
+
+        const Runs r1{T.side_run + T.side_run_off[mv.bond >= 0 ? mv.bond : 0],
+                      mv.bond >= 0 ? T.side_run_off[mv.bond + 1] - T.side_run_off[mv.bond] : 0};
+        const Runs r2{T.side_run + T.side_run_off[mv.bond2 >= 0 ? mv.bond2 : 0],
+                      (mv.kind == 2 && mv.bond2 >= 0) ? T.side_run_off[mv.bond2 + 1] - T.side_run_off[mv.bond2]
+                                                      : 0};
+        const int n1 = mv.valid ? r1.count() : 0;
+        const int n2 = (mv.valid && mv.kind == 2) ? r2.count() : 0;
+
+        // ---- trial coordinates of the moved atoms
+        if (mv.valid) {
+            for (int k = tid; k < n1; k += kMcBlock) {
+                int i = r1.at(k);
+                rodrigues_d(pos + 4 * i, mv.o, mv.u, mv.cs, mv.sn, posn + 4 * i);
+            }
+            for (int k = tid; k < n2; k += kMcBlock) {
+                int i = r2.at(k);
+                rodrigues_d(pos + 4 * i, mv.o2, mv.u2, mv.cs2, mv.sn2, posn + 4 * i);
+            }
+        }
+        if (tid == 0) far_move = 0;
+        __syncthreads();
+        if (mv.valid && mv.kind == 1) {
+            // crankshaft: psi axis from the rotated coordinates, then the psi
+            // side turned back by -delta (src/physics_engine.cpp:2451-2470)
+            if (tid == 0) {
+                if (!axis_of(posn, T, mv.bond2, mv.o2, mv.u2)) mv.valid = 0;
+            }
+            __syncthreads();
+            if (mv.valid) {
+                const Runs rp{T.side_run + T.side_run_off[mv.bond2],
+                              T.side_run_off[mv.bond2 + 1] - T.side_run_off[mv.bond2]};
+                int np = rp.count();
+                for (int k = tid; k < np; k += kMcBlock) {
+                    int i = rp.at(k);
+                    double q[3] = {posn[4 * i], posn[4 * i + 1], posn[4 * i + 2]};
+                    rodrigues_d(q, mv.o2, mv.u2, mv.cs, -mv.sn, posn + 4 * i);
+                }
+            }
+            __syncthreads();
+        }
+
+        if (mv.valid) {
+            // enumeration set: the moved set or its complement, whichever is
+            // smaller (mode 0 / 1); both groups of a concerted move (mode 2)
+            const int mode = mv.kind == 2 ? 2 : (n1 <= n - n1 ? 0 : 1);
+            const int nT = mode == 0 ? n1 : mode == 1 ? n - n1 : n1 + n2;
+            auto group = [&](int i) -> int { return r1.has(i) ? 1 : (r2.n && r2.has(i) ? 2 : 0); };
+            auto elem = [&](int k) -> int {
+                if (mode == 0) return r1.at(k);
+                if (mode == 1) return r1.gap_at(k);
+                return k < n1 ? r1.at(k) : r2.at(k - n1);
+            };
+
+            // visit every cross pair (i from the enumeration set, j partner)
+            auto for_cross = [&](auto&& fn) {
+                for (int k = wave; k < nT; k += kMcWaves) {
+                    int i = elem(k);
+                    int gi = mode == 0 ? 1 : mode == 1 ? 0 : group(i);
+                    const uint32_t* row = nlf + (size_t)i * b.cap_f;
+                    int cnt = min(nlf_cnt[i], b.cap_f);
+                    if (mode == 2) {
+                        for (int p = lane; p < cnt; p += kWave) {
+                            uint32_t e = row[p];
+                            int j = (int)(e & kIdxMask);
+                            int gj = group(j);
+                            if (gj == gi || (gj != 0 && j < i)) continue;
+                            fn(i, gi, e, gj, true);
+                        }
+                        fn(i, gi, 0u, 0, false);
+                        continue;
+                    }
+                    // partner ranges: outside the runs (i moved) or inside them
+                    for (int q = 0; q <= r1.n; ++q) {
+                        int p0, p1;
+                        if (mode == 0) {
+                            p0 = q == 0 ? 0 : lower_bound_row(row, 0, cnt, r1.r[q - 1].y + 1);
+                            p1 = q == r1.n ? cnt : lower_bound_row(row, p0, cnt, r1.r[q].x);
+                        } else {
+                            if (q == r1.n) break;
+                            p0 = lower_bound_row(row, 0, cnt, r1.r[q].x);
+                            p1 = lower_bound_row(row, p0, cnt, r1.r[q].y + 1);
+                        }
+                        for (int p = p0 + lane; p < p1; p += kWave) {
+                            uint32_t e = row[p];
+                            fn(i, gi, e, 1 - gi, true);
+                        }
+                    }
+                    fn(i, gi, 0u, 0, false);
+                }
+            };
+
+            // ---- pass A: HCT change of every cross pair (half-row owner =
+            // lower index), marks atoms whose SASA fold changes
+            {
+                double own = 0.0;
+                bool own_hit = false;
+                for_cross([&](int i, int gi, uint32_t e, int gj, bool is_pair) {
+                    if (!is_pair) {   // end of atom i: flush its own accumulator
+                        double v = wave_sum(own);
+                        bool any = __ballot(own_hit) != 0ull;
+                        if (lane == 0 && any) {
+                            atomicAdd(&dhalf[i], v);
+                            flag[i] = 1;
+                        }
+                        own = 0.0;
+                        own_hit = false;
+                        return;
+                    }
+                    int j = (int)(e & kIdxMask);
+                    const double* pio = pos + 4 * i;
+                    const double* pjo = pos + 4 * j;
+                    const double* pin = gi ? posn + 4 * i : pio;
+                    const double* pjn = gj ? posn + 4 * j : pjo;
+                    float r2o = (float)dist2_exact(pio, pjo), r2n = (float)dist2_exact(pin, pjn);
+                    int lo = min(i, j), hi = max(i, j);
+                    float rl = s.prm[lo].y, rh = s.prm[hi].y;
+                    double dlt = (double)hct_f(sqrtf(r2n), r2n, rl, rh) - (double)hct_f(sqrtf(r2o), r2o, rl, rh);
+                    if (lo == i) {
+                        own += dlt;
+                        own_hit = true;
+                    } else {
+                        atomicAdd(&dhalf[j], dlt);
+                        flag[j] = 1;
+                    }
+                });
+            }
+            if (mv.kind == 1) {
+                // crankshaft: atoms of the phi side outside the psi side turn
+                // once, the psi side twice, so pairs between the two parts
+                // change length.  cross_e of the reference ignores them (both
+                // atoms are in_side) but update_born and the full SASA
+                // recomputation see them: feed them to the Born and SASA
+                // bookkeeping only.
+                const Runs rp{T.side_run + T.side_run_off[mv.bond2],
+                              T.side_run_off[mv.bond2 + 1] - T.side_run_off[mv.bond2]};
+                for (int k = wave; k < n1; k += kMcWaves) {
+                    int i = r1.at(k);
+                    if (rp.has(i)) continue;
+                    const uint32_t* row = nlf + (size_t)i * b.cap_f;
+                    int cnt = min(nlf_cnt[i], b.cap_f);
+                    double own = 0.0;
+                    bool own_hit = false;
+                    for (int q = 0; q < rp.n; ++q) {
+                        int p0 = lower_bound_row(row, 0, cnt, rp.r[q].x);
+                        int p1 = lower_bound_row(row, p0, cnt, rp.r[q].y + 1);
+                        for (int p = p0 + lane; p < p1; p += kWave) {
+                            int j = (int)(row[p] & kIdxMask);
+                            float r2o = (float)dist2_exact(pos + 4 * i, pos + 4 * j);
+                            float r2n = (float)dist2_exact(posn + 4 * i, posn + 4 * j);
+                            int lo = min(i, j), hi = max(i, j);
+                            float rl = s.prm[lo].y, rh = s.prm[hi].y;
+                            double dlt = (double)hct_f(sqrtf(r2n), r2n, rl, rh) -
+                                         (double)hct_f(sqrtf(r2o), r2o, rl, rh);
+                            if (lo == i) {
+                                own += dlt;
+                                own_hit = true;
+                            } else {
+                                atomicAdd(&dhalf[j], dlt);
+                                flag[j] = 1;
+                            }
+                        }
+                    }
+                    double v = wave_sum(own);
+                    bool any = __ballot(own_hit) != 0ull;
+                    if (lane == 0 && any) {
+                        atomicAdd(&dhalf[i], v);
+                        flag[i] = 1;
+                    }
+                }
+            }
+            __syncthreads();
+            // ---- trial radii of moved atoms: update_born on the half row;
+            // largest trial displacement from the build positions
+            {
+                bool far = false;
+                for (int k = tid; k < n1 + n2; k += kMcBlock) {
+                    int i = k < n1 ? r1.at(k) : r2.at(k - n1);
+                    anew[i] = 1.0 / fmax(1.0 / s.rstar[i] - 0.5 * (half[i] + dhalf[i]), 2.0);
+                    double d2 = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        double dd = posn[4 * i + d] - ref[4 * i + d];
+                        d2 += dd * dd;
+                    }
+                    far |= d2 > kSasaTrial * kSasaTrial;
+                }
+                if (far) far_move = 1;
+            }
+            __syncthreads();
+            // ---- pass B: cross-pair energy change, SASA, dihedral, SS
+            double acc = 0.0, acc_sa = 0.0, acc_dh = 0.0, acc_ss = 0.0;
+            for_cross([&](int i, int gi, uint32_t e, int gj, bool is_pair) {
+                if (!is_pair || (e & kExclBit)) return;
+                int j = (int)(e & kIdxMask);
+                const double* pio = pos + 4 * i;
+                const double* pjo = pos + 4 * j;
+                const double* pin = gi ? posn + 4 * i : pio;
+                const double* pjn = gj ? posn + 4 * j : pjo;
+                double r2o = dist2_exact(pio, pjo), r2n = dist2_exact(pin, pjn);
+                float4 qi = s.prm[i], qj = s.prm[j];
+                const double Ri = s.rstar[i], Rj = s.rstar[j], ei = s.eps64[i], ej = s.eps64[j];
+                float4 oi = qi, oj = qj, ni = qi, nj = qj;
+                oi.w = (float)a[i];
+                oj.w = (float)a[j];
+                ni.w = (float)(gi ? anew[i] : a[i]);
+                nj.w = (float)(gj ? anew[j] : a[j]);
+                bool in_o = r2o <= kPairCut2, in_n = r2n <= kPairCut2;
+                if (in_o && in_n)
+                    acc += pair_delta(r2o, r2n, oi, oj, ni, nj, Ri, Rj, ei, ej);
+                else if (in_o)
+                    acc -= pair_energy(r2o, oi, oj, Ri, Rj, ei, ej);
+                else if (in_n)
+                    acc += pair_energy(r2n, ni, nj, Ri, Rj, ei, ej);
+            });
+            for (int i = tid; i < n; i += kMcBlock) {
+                if (!flag[i]) continue;
+                double pi_n[3];
+                const double* pi = pos + 4 * i;
+                bool gi = group(i) != 0;
+                // new fold: moved atoms at trial coordinates
+                double ri = (double)s.prm[i].y + kProbe;
+                double sa = 4.0 * M_PI * ri * ri;
+                const double k2 = M_PI * ri;
+                for (int d = 0; d < 3; ++d) pi_n[d] = gi ? posn[4 * i + d] : pi[d];
+                int m;
+                const uint32_t* row;
+                if (far_move) {   // trial move beyond the sub-row's reach: whole half row
+                    m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
+                    row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+                } else {
+                    m = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
+                    row = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
+                }
+                for (int k = 0; k < m; ++k) {
+                    int j = (int)(row[k] & kIdxMask);
+                    const double* pj = group(j) ? posn + 4 * j : pos + 4 * j;
+                    double dc = ri + (double)s.prm[j].y + kProbe;
+                    double r2 = dist2_exact(pi_n, pj);
+                    if (r2 >= dc * dc) continue;
+                    double r = (double)sqrtf((float)r2);
+                    if (r >= dc) continue;
+                    sa -= fmin(sa * 0.85, k2 * (dc - r));
+                }
+                acc_sa += kGammaSA * fmax(0.0, sa) - sasa_fold(b, s, c, i, pi, pos);
+            }
+            if (s.has_topo) {
+                for (int k = tid; k < T.n_dih; k += kMcBlock) {
+                    int q4[4] = {T.dih_a[k], T.dih_b[k], T.dih_c[k], T.dih_d[k]};
+                    bool any_m = false, any_f = false;
+                    for (int m = 0; m < 4; ++m) (group(q4[m]) ? any_m : any_f) = true;
+                    if (!any_m || !any_f) continue;
+                    double po[16], pn[16];
+                    for (int m = 0; m < 4; ++m)
+                        for (int d = 0; d < 4; ++d) {
+                            po[4 * m + d] = pos[4 * q4[m] + d];
+                            pn[4 * m + d] = group(q4[m]) ? posn[4 * q4[m] + d] : pos[4 * q4[m] + d];
+                        }
+                    double fo = dih_angle_d(po, 0, 1, 2, 3), fn = dih_angle_d(pn, 0, 1, 2, 3);
+                    for (int m = T.dih_toff[k]; m < T.dih_toff[k + 1]; ++m) {
+                        double nn = (double)T.dih_n[m];
+                        acc_dh += T.dih_v2[m] * (cos(nn * fn - T.dih_gamma[m]) - cos(nn * fo - T.dih_gamma[m]));
+                    }
+                }
+                for (int k = tid; k < T.n_ss; k += kMcBlock) {
+                    int i = T.ss_i[k], j = T.ss_j[k];
+                    int gi = group(i), gj = group(j);
+                    if (!gi && !gj) continue;
+                    double dro = sqrt(dist2_exact(pos + 4 * i, pos + 4 * j)) - kR0ss;
+                    double drn = sqrt(dist2_exact(gi ? posn + 4 * i : pos + 4 * i,
+                                                  gj ? posn + 4 * j : pos + 4 * j)) - kR0ss;
+                    acc_ss += kKss * (drn * drn - dro * dro);
+                }
+            }
+            acc = wave_sum(acc);
+            acc_sa = wave_sum(acc_sa);
+            acc_dh = wave_sum(acc_dh);
+            acc_ss = wave_sum(acc_ss);
+            if (lane == 0) {
+                red[wave] = acc;
+                red[kMcWaves + wave] = acc_sa;
+                red[2 * kMcWaves + wave] = acc_dh;
+                red[3 * kMcWaves + wave] = acc_ss;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double part[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int q = 0; q < 4; ++q)
+                    for (int w = 0; w < kMcWaves; ++w) part[q] += red[q * kMcWaves + w];
+                double dE = part[0] + part[1] + part[2] + part[3];
+                for (int q = 0; q < 4; ++q) mv.part[q] = part[q];
+                mv.dE = dE;
+                bool acc_ok = dE < 0.0 || W.canonical() < exp(-dE / cs->temperature);
+                if (W.bad) {
+                    acc_ok = false;
+                    cs->status = CS_ERROR;
+                }
+                mv.accept = acc_ok ? 1 : 0;
+                if (acc_ok) cs->cur_e += dE;
+            }
+            __syncthreads();
+            // ---- commit or discard
+            if (mv.accept) {
+                for (int k = tid; k < n1 + n2; k += kMcBlock) {
+                    int i = k < n1 ? r1.at(k) : r2.at(k - n1);
+                    double d2 = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        pos[4 * i + d] = posn[4 * i + d];
+                        double dd = pos[4 * i + d] - ref[4 * i + d];
+                        d2 += dd * dd;
+                    }
+                    a[i] = anew[i];
+                    if (d2 > kHalfSkin2) mv.rebuild = 1;
+                }
+            }
+            for (int i = tid; i < n; i += kMcBlock) {
+                if (!flag[i]) continue;
+                if (mv.accept) half[i] += dhalf[i];
+                dhalf[i] = 0.0;
+                flag[i] = 0;
+            }
+            __syncthreads();
+        }
+
+        // ---- bookkeeping (tid 0)
+        if (tid == 0) {
+            int step = cs->steps_done;
+            if (trace && step < b.trace_cap) {
+                b.tr_kind[step] = mv.kind;
+                b.tr_bond[step] = mv.kind == 1 || mv.kind == 2 ? -1 : mv.bond;
+                b.tr_delta[step] = mv.delta;
+                b.tr_dE[step] = mv.dE;
+                for (int q = 0; q < 4; ++q) b.tr_parts[4 * step + q] = mv.part[q];
+                b.tr_acc[step] = mv.accept;
+            }
+            if (mv.valid) {
+                cs->moves += 1;
+                if (mv.accept) cs->accepted += 1;
+                if (cs->mode != ALMA_MC_TORSION30) {
+                    // adaptive step, src/physics_engine.cpp:2519-2527
+                    cs->acc_win += mv.accept;
+                    if (++cs->tot_win == 200) {
+                        double rate = (double)cs->acc_win / 200.0;
+                        if (rate > 0.52)
+                            cs->cur_max = fmin(0.50, cs->cur_max * 1.06);
+                        else if (rate < 0.28)
+                            cs->cur_max = fmax(0.004, cs->cur_max * 0.94);
+                        cs->acc_win = cs->tot_win = 0;
+                    }
+                }
+            }
+            cs->steps_done = step + 1;
+            cs->cursor = W.cur;
+            if (cs->status == CS_ERROR) mv.stop = 1;
+            if (mv.rebuild) {
+                cs->status = CS_REBUILD;
+                mv.stop = 1;
+            }
+        }
+        __syncthreads();
+        if (mv.stop) break;
+    }
+    if (tid == 0) cs->cursor = W.cur;
+}
+
+inline unsigned blocks_for(int n, int per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+hipError_t launch_chain_rebuild(ChainBatch& b, const DevSys& s, int L, int count, hipStream_t st) {
+    if (L <= 0) return hipSuccess;
+    const double cs = std::sqrt(kNlRcut2) + 1e-6;
+    dim3 g1(1, L), ga(blocks_for(b.n, kBlock), L), gw(blocks_for(b.n, kWavesPerBlock), L);
+    rb_clear<<<dim3(16, L), kBlock, 0, st>>>(b);
+    rb_bbox<<<g1, kBlock, 0, st>>>(b, cs);
+    rb_cell_assign<<<ga, kBlock, 0, st>>>(b);
+    rb_cell_scan<<<g1, 1024, 0, st>>>(b);
+    rb_cell_place<<<ga, kBlock, 0, st>>>(b);
+    rb_cell_rank<<<ga, kBlock, 0, st>>>(b);
+    rb_nl_fill<<<gw, kBlock, 0, st>>>(b, s);
+    rb_full_sort<<<dim3(b.n, L), kSortBlock, sizeof(uint32_t) * b.cap_f, st>>>(b);
+    rb_born_sum<<<gw, kBlock, 0, st>>>(b, s);
+    rb_born_fin<<<ga, kBlock, 0, st>>>(b, s);
+    rb_pair<<<gw, kBlock, 0, st>>>(b, s);
+    rb_sasa<<<ga, kBlock, 0, st>>>(b, s);
+    rb_total<<<g1, kBlock, 0, st>>>(b, s, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int L, hipStream_t st) {
+    if (L <= 0) return hipSuccess;
+    k_mc<<<L, kMcBlock, 0, st>>>(b, s);
+    return hipGetLastError();
+}
+
+}  // namespace alma

@@ -1,0 +1,158 @@
+// physics_dev.h -- device-side arithmetic shared by the full-evaluation
+// kernels (energy.hip), the MC kernels (mc.hip) and the force kernels
+// (forces.hip).  Formulas follow src/physics_engine.cpp of the reference at
+// the cited lines.
+#pragma once
+#include "alma_internal.h"
+
+namespace alma {
+namespace dev {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;            // 4 wavefronts
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Reference FP64 squared distance, no contraction into FMA (the reference's
+// d^2 < 196 and d^2 > 144 decisions must be reproduced exactly).
+__device__ __forceinline__ double dist2_exact(const double* pi, const double* pj) {
+#pragma clang fp contract(off)
+    double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ bool in_sorted(const int* __restrict__ v, int b, int e, int key) {
+    while (b < e) {
+        int m = (b + e) >> 1;
+        int x = v[m];
+        if (x == key) return true;
+        if (x < key)
+            b = m + 1;
+        else
+            e = m;
+    }
+    return false;
+}
+
+// ── HCT Born radii ──────────────────────────────────────────────────────────
+// hct, src/physics_engine.cpp:2007-2011 (as written, FP32);
+// born_radii :2024-2039 sums both directions over every neighbour entry.
+__device__ __forceinline__ float hct_f(float r, float r2, float ri, float rj) {
+    float L = fmaxf(fabsf(r - rj), ri), U = r + rj;
+    if (ri >= U) return 0.0f;
+    return 1.0f / L - 1.0f / U + (r2 - rj * rj + ri * ri) / (2.0f * r * ri * ri) * logf(L / U) * 0.5f / r;
+}
+
+// ── fused pair energy ──────────────────────────────────────────────────────
+// pair_e / pair_e_diag, src/physics_engine.cpp:2148-2199, split into DH, GB,
+// LJ and hard-core.  DH and GB are FP32.  LJ is FP32 outside sigma; inside
+// sigma (the steep wall, where a pair can carry 1e3-1e5 kcal/mol and FP32
+// sigma / epsilon / r^2 rounding alone would cost ~4e-7 of it) LJ and the
+// capped hard-core term are FP64 from the FP64 R*, epsilon and r^2.
+struct PairTerms {
+    double dh, gb, lj, hc;
+};
+
+__device__ __forceinline__ PairTerms pair_terms(double r2d, float4 pi, float4 pj, double Ri, double Rj,
+                                                double ei, double ej) {
+    PairTerms t;
+    float r2 = (float)r2d;
+    float r = sqrtf(r2);
+    float qp = pi.x * pj.x;
+    t.dh = (double)(((float)(kCoulomb / kEpsWater) * qp / r) * expf(-(float)kKappa * r));
+    float D = pi.w * pj.w;
+    float fgb = sqrtf(r2 + D * expf(-r2 / (4.0f * D)));
+    t.gb = (double)((float)kGbCoef * qp / fgb);
+    double sig = Ri + Rj;
+    double s2d = sig * sig / r2d;
+    t.hc = 0.0;
+    if (s2d > 1.0) {
+        double s6 = s2d * s2d * s2d;
+        t.lj = 4.0 * sqrt(ei * ej) * (s6 * s6 - s6);
+        double rc = sig * kHardFrac;
+        if (r2d < rc * rc) {
+            double c2 = rc * rc / r2d;
+            double c6 = c2 * c2 * c2;
+            double x = c6 * c6 - 1.0;
+            t.hc = fmin(kHardScale * x * x, kHardCap);
+        }
+    } else {
+        float eps = sqrtf(pi.z * pj.z);
+        float s2 = (float)s2d;
+        float s6 = s2 * s2 * s2;
+        t.lj = (double)(4.0f * eps * (s6 * s6 - s6));
+    }
+    return t;
+}
+
+__device__ __forceinline__ double pair_energy(double r2d, float4 pi, float4 pj, double Ri, double Rj,
+                                              double ei, double ej) {
+    PairTerms t = pair_terms(r2d, pi, pj, Ri, Rj, ei, ej);
+    return t.dh + t.gb + t.lj + t.hc;
+}
+
+// Change of one pair's energy between an old and a new geometry (and Born
+// radii), both inside the cutoff.  The difference is formed before rounding
+// wherever the two energies nearly cancel: r_new^2 - r_old^2 comes from FP64
+// coordinates, and
+//   dGB = C qq (f_o^2 - f_n^2) / (f_o f_n (f_o + f_n)),
+//   dDH = K qq e^{-k r_o} (r_o expm1(-k dr) - dr) / (r_o r_n),
+// so the FP32 error is relative to the change, not to the energies.
+// LJ / hard-core use pair_terms (FP64 inside sigma, small outside).
+__device__ __forceinline__ double pair_delta(double r2o, double r2n, float4 po_i, float4 po_j,
+                                             float4 pn_i, float4 pn_j, double Ri, double Rj, double ei,
+                                             double ej) {
+    float ro2 = (float)r2o, rn2 = (float)r2n;
+    float ro = sqrtf(ro2), rn = sqrtf(rn2);
+    float d2 = (float)(r2n - r2o);
+    float dr = d2 / (ro + rn);
+    float qp = po_i.x * po_j.x;
+    // Debye-Hueckel
+    float dh = (float)(kCoulomb / kEpsWater) * qp * expf(-(float)kKappa * ro) *
+               (ro * expm1f(-(float)kKappa * dr) - dr) / (ro * rn);
+    // generalized Born
+    float Do = po_i.w * po_j.w, Dn = pn_i.w * pn_j.w;
+    float go = Do * expf(-ro2 / (4.0f * Do)), gn = Dn * expf(-rn2 / (4.0f * Dn));
+    float fo = sqrtf(ro2 + go), fn = sqrtf(rn2 + gn);
+    float gb = (float)kGbCoef * qp * ((-d2) + (go - gn)) / (fo * fn * (fo + fn));
+    // Lennard-Jones + hard core
+    PairTerms to = pair_terms(r2o, po_i, po_j, Ri, Rj, ei, ej);
+    PairTerms tn = pair_terms(r2n, pn_i, pn_j, Ri, Rj, ei, ej);
+    return (double)dh + (double)gb + (tn.lj - to.lj) + (tn.hc - to.hc);
+}
+
+// ── dihedral / disulfide (FP64) ─────────────────────────────────────────────
+// dihedral_angle :1871-1885 (atan2 of the reference's non-IUPAC pair),
+// dihedral_e :1912-1921, ss_e :1965-1974.
+__device__ inline double dih_angle_d(const double* pos, int a, int b, int c, int d) {
+    double b1[3], b2[3], b3[3];
+    for (int k = 0; k < 3; ++k) {
+        b1[k] = pos[4 * a + k] - pos[4 * b + k];
+        b2[k] = pos[4 * c + k] - pos[4 * b + k];
+        b3[k] = pos[4 * d + k] - pos[4 * c + k];
+    }
+    double n1x = b1[1] * b2[2] - b1[2] * b2[1], n1y = b1[2] * b2[0] - b1[0] * b2[2],
+           n1z = b1[0] * b2[1] - b1[1] * b2[0];
+    double n2x = b2[1] * b3[2] - b2[2] * b3[1], n2y = b2[2] * b3[0] - b2[0] * b3[2],
+           n2z = b2[0] * b3[1] - b2[1] * b3[0];
+    double m1x = n1y * b2[2] - n1z * b2[1], m1y = n1z * b2[0] - n1x * b2[2],
+           m1z = n1x * b2[1] - n1y * b2[0];
+    return atan2(m1x * n2x + m1y * n2y + m1z * n2z, n1x * n2x + n1y * n2y + n1z * n2z);
+}
+
+
+}  // namespace dev
+}  // namespace alma

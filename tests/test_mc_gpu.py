@@ -1,0 +1,121 @@
+"""GPU Monte Carlo parity against the FP64 oracle (seeded mt19937 streams).
+
+Each chain draws std::mt19937(seed + chain) words with libstdc++ distribution
+semantics, so the oracle replays the same proposals.  Checked per step: the
+move kind, dE within 1e-5 * sum|per-item change| (pairs, per-atom SASA,
+dihedral records, SS pairs), and the accept decision; at the end the RNG
+words consumed, the rebuild count, the running energy and the coordinates.
+A step whose Metropolis test sits within the dE error of its threshold may
+flip; the tests require none to flip over these seeds.
+"""
+import numpy as np
+import pytest
+
+import systems as S
+from alma import _native as N
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KT = 0.0019872041 * 300.0
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+def _load(ctx, s, c):
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+
+
+def _compare(gpu, orc, steps, tag):
+    gt, ot = gpu["trace"], orc["trace"]
+    assert np.array_equal(gt["kind"], ot["kind"]), tag
+    mism = np.nonzero(gt["accepted"] != ot["accepted"])[0]
+    assert mism.size == 0, (tag, "first flip at step", int(mism[0]) if mism.size else None)
+    tol = 1e-5 * ot["dE_l1"] + 1e-9
+    err = np.abs(gt["dE"] - ot["dE"])
+    bad = np.nonzero(err > tol)[0]
+    assert bad.size == 0, (tag, int(bad[0]), gt["dE"][bad[0]], ot["dE"][bad[0]], tol[bad[0]])
+    gs, os_ = gpu["stats"][0], orc["stats"]
+    assert gs["words"] == os_["words"], tag
+    assert gs["rebuilds"] == os_["rebuilds"], tag
+    assert gs["accepted"] == os_["accepted"] and gs["moves"] == os_["moves"], tag
+    assert abs(gs["cur_e"] - os_["cur_e"]) <= 1e-6 * max(1.0, abs(os_["cur_e"])), tag
+    assert abs(gs["cur_max"] - os_["cur_max"]) <= 1e-15, tag
+    assert np.max(np.abs(gpu["xyz"][0] - orc["xyz"])) < 1e-8, tag
+
+
+@pytest.mark.parametrize("mode,steps", [(N.MC_ENSEMBLE, 300), (N.MC_TORSION30, 200),
+                                        (N.MC_LANDSCAPE, 300)])
+def test_trajectory_matches_oracle(ctx, mode, steps):
+    s = S.chain(56, 1)
+    t = S.topology(s)
+    if mode == N.MC_LANDSCAPE:
+        t.identify_concerted_sidechain_pairs(S.particles(s))
+    c = S.csr(t)
+    _load(ctx, s, c)
+    gpu = ctx.mc_run(mode, steps, n_chains=1, temperature=KT, max_angle=0.12, seed=1234, trace=True)
+    orc = O.mc_run(s, c, steps, KT, 0.12, seed=1234, mode=mode, trace=True, l1=True)
+    _compare(gpu, orc, steps, "mode%d" % mode)
+
+
+def test_chains_use_consecutive_seeds(ctx):
+    s = S.chain(30, 5)
+    t = S.topology(s)
+    c = S.csr(t)
+    _load(ctx, s, c)
+    gpu = ctx.mc_run(N.MC_ENSEMBLE, 150, n_chains=3, temperature=KT, seed=77)
+    for k in range(3):
+        orc = O.mc_run(s, c, 150, KT, 0.12, seed=77 + k)
+        assert gpu["stats"][k]["words"] == orc["stats"]["words"]
+        assert np.max(np.abs(gpu["xyz"][k] - orc["xyz"])) < 1e-8
+
+
+def test_disulfide_system_mc(ctx):
+    s = S.two_chains_ss()
+    sg = [i for i, n in enumerate(s["atomnames"]) if n == "SG"]
+    t = S.topology(s, [(sg[0], sg[1])])
+    c = S.csr(t)
+    _load(ctx, s, c)
+    gpu = ctx.mc_run(N.MC_ENSEMBLE, 200, temperature=KT, seed=9, trace=True)
+    orc = O.mc_run(s, c, 200, KT, 0.12, seed=9, trace=True, l1=True)
+    _compare(gpu, orc, 200, "ss")
+
+
+def test_landscape_snapshots_fresh_energy(ctx):
+    s = S.chain(40, 8)
+    t = S.topology(s)
+    t.identify_concerted_sidechain_pairs(S.particles(s))
+    c = S.csr(t)
+    _load(ctx, s, c)
+    gpu = ctx.mc_run(N.MC_LANDSCAPE, 120, temperature=KT, seed=5, snapshot_every=40)
+    orc = O.mc_run(s, c, 120, KT, 0.12, seed=5, mode=O.MODE_LANDSCAPE, snapshot_every=40)
+    assert np.max(np.abs(gpu["snapshots"][0] - orc["snapshots"])) < 1e-8
+    rel = np.abs(gpu["energies"][0] - orc["energies"]) / np.abs(orc["energies"])
+    assert rel.max() <= 1e-5
+
+
+def test_mc_rejects_bad_arguments(ctx):
+    s = S.chain(12, 2)
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"])   # no topology
+    with pytest.raises(ValueError):
+        ctx.mc_run(N.MC_ENSEMBLE, 10)
+    _load(ctx, s, S.csr(S.topology(s)))
+    with pytest.raises(ValueError):
+        ctx.mc_run(N.MC_ENSEMBLE, 0)
+
+
+def test_dropin_generate_ensemble_seeded(ctx):
+    import protein_physics as PP
+    s = S.chain(30, 5)
+    t = S.topology(s)
+    eng = PP.PhysicsEngine()
+    ens = eng.generate_ensemble(S.particles(s), t, 2, 100, KT, 0.12, seed=21)
+    assert len(ens) == 2 and len(ens[0]) == len(s["xyz"])
+    orc = O.mc_run(s, S.csr(t), 100, KT, 0.12, seed=22)
+    got = np.array([[p.x, p.y, p.z] for p in ens[1]])
+    assert np.max(np.abs(got - orc["xyz"])) < 1e-8
