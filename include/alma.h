@@ -123,6 +123,13 @@ alma_status alma_mc_run(alma_ctx* ctx, const alma_mc_params* p, double* xyz_out,
 alma_status alma_mc_trace(alma_ctx* ctx, int32_t* kind, int32_t* bond, double* delta,
                           double* dE, int32_t* accepted, double* dE_parts);
 
+/* Measurement of the last alma_mc_run (no reference counterpart): out[9] =
+ * {device MC kernel ms (HIP events on the context stream), MC launches,
+ *  list-rebuild ms, rebuild batches, cross pairs, in-cutoff pair
+ *  evaluations, SASA row entries folded, moved atoms, boundary dihedral
+ *  records}, summed over chains. */
+alma_status alma_mc_timing(alma_ctx* ctx, double* out);
+
 /* ---- forces (no reference counterpart) ---- */
 /* E, dE/dx (3n) with the Born-radius chain rule, dE/dtheta per rotatable
  * bond (torque about each bond for its j-side), at the loaded positions. */

@@ -69,13 +69,14 @@ def lib():
         L.alma_mc_run.argtypes = [P, C.POINTER(McParams), _D, C.POINTER(McStats), _D, _D]
         L.alma_mc_trace.argtypes = [P, _I, _I, _D, _D, _I, _D]
         L.alma_forces.argtypes = [P, _D, _D, _D]
+        L.alma_mc_timing.argtypes = [P, _D]
         _LIB = L
     return _LIB
 
 
 EXPORTED = ("alma_create", "alma_destroy", "alma_last_error", "alma_device_name",
             "alma_set_system", "alma_set_positions", "alma_energy", "alma_nlist",
-            "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_forces")
+            "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_mc_timing", "alma_forces")
 
 
 def _p(a, t):
@@ -210,6 +211,13 @@ class Context:
                                             _p(tr["parts"], C.c_double)))
             out["trace"] = tr
         return out
+
+    def mc_timing(self) -> dict:
+        out = np.zeros(9)
+        self._check(lib().alma_mc_timing(self._h, _p(out, C.c_double)))
+        keys = ("mc_kernel_ms", "mc_launches", "rebuild_ms", "rebuild_batches", "cross_pairs",
+                "pair_evals", "sasa_entries", "moved_atoms", "dih_records")
+        return dict(zip(keys, out.tolist()))
 
     def forces(self):
         e = C.c_double()

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -316,6 +318,7 @@ void free_chains(ChainBatch& b) {
     dev_free(b.words);
     dev_free(b.st);
     dev_free(b.list);
+    dev_free(b.seg);
     dev_free(b.tr_kind);
     dev_free(b.tr_bond);
     dev_free(b.tr_delta);
@@ -362,6 +365,7 @@ void alloc_chains(ChainBatch& b, const DevSys& s, int nch, int trace_steps) {
     dev_alloc(&b.words, (size_t)nch * b.wcap);
     dev_alloc(&b.st, nch);
     dev_alloc(&b.list, nch);
+    dev_alloc(&b.seg, N * 4);
     HIP_TRY(hipMemset(b.dhalf, 0, sizeof(double) * N));
     HIP_TRY(hipMemset(b.flag, 0, sizeof(int) * N));
     if (trace_steps > 0) {
@@ -428,6 +432,12 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     }
     std::vector<ChainState> st(nch);
     std::vector<HostMT> mt;
+    ctx->t_mc_ms = ctx->t_rebuild_ms = 0.0;
+    ctx->n_mc_launch = ctx->n_rebuild_launch = 0;
+    if (!ctx->ev0) {
+        HIP_TRY(hipEventCreate(&ctx->ev0));
+        HIP_TRY(hipEventCreate(&ctx->ev1));
+    }
     std::vector<std::vector<uint32_t>> wbuf(nch, std::vector<uint32_t>(b.wcap));
     std::vector<long long> consumed(nch, 0);
     mt.reserve(nch);
@@ -512,10 +522,48 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
         }
         if (run.empty()) break;
         if (dirty) HIP_TRY(hipMemcpy(b.st, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice));
-        if (!rb.empty()) rebuild_chains(ctx, rb, 1);
+        if (!rb.empty()) {
+            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+            rebuild_chains(ctx, rb, 1);
+            HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+            HIP_TRY(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            ctx->t_rebuild_ms += ms;
+            ctx->n_rebuild_launch += 1;
+        }
         HIP_TRY(hipMemcpyAsync(b.list, run.data(), sizeof(int) * run.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
         HIP_TRY(launch_chain_mc(b, s, (int)run.size(), ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev1));
+        {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            ctx->t_mc_ms += ms;
+            ctx->n_mc_launch += 1;
+        }
+    }
+    for (int k = 0; k < 5; ++k) ctx->work[k] = 0;
+    for (int c = 0; c < nch; ++c) {
+        ctx->work[0] += st[c].n_cross;
+        ctx->work[1] += st[c].n_pair_eval;
+        ctx->work[2] += st[c].n_sasa;
+        ctx->work[3] += st[c].n_moved;
+        ctx->work[4] += st[c].n_dih;
+    }
+    if (std::getenv("ALMA_MC_PROFILE")) {
+        ChainState& x = st[0];
+        long long tot = 0;
+        for (int k = 0; k < 7; ++k) tot += x.clk[k];
+        std::fprintf(stderr,
+                     "[alma mc] chain0 steps=%d torsion=%lld crank=%lld conc=%lld  us/step: select %.2f move %.2f "
+                     "segments %.2f passA %.2f radii %.2f passB %.2f commit %.2f total %.2f\n",
+                     x.steps_done, x.steps_by_kind[0], x.steps_by_kind[1], x.steps_by_kind[2],
+                     x.clk[0] / 100.0 / x.steps_done, x.clk[1] / 100.0 / x.steps_done,
+                     x.clk[2] / 100.0 / x.steps_done, x.clk[3] / 100.0 / x.steps_done,
+                     x.clk[4] / 100.0 / x.steps_done, x.clk[5] / 100.0 / x.steps_done,
+                     x.clk[6] / 100.0 / x.steps_done, tot / 100.0 / x.steps_done);
     }
     // results
     std::vector<double> p4(4 * (size_t)n);
@@ -582,6 +630,8 @@ void alma_destroy(alma_ctx* ctx) {
     hipSetDevice(ctx->device);
     free_sys(ctx->sys);
     free_chains(ctx->chains);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -723,6 +773,17 @@ alma_status alma_mc_trace(alma_ctx* ctx, int32_t* kind, int32_t* bond, double* d
         if (dE) std::copy(ctx->tr_dE.begin(), ctx->tr_dE.end(), dE);
         if (accepted) std::copy(ctx->tr_acc.begin(), ctx->tr_acc.end(), accepted);
         if (dE_parts) std::copy(ctx->tr_parts.begin(), ctx->tr_parts.end(), dE_parts);
+    });
+}
+
+alma_status alma_mc_timing(alma_ctx* ctx, double* out) {
+    return guarded(ctx, [&] {
+        if (!out) throw Fail{ALMA_EINVAL, "output buffer missing"};
+        out[0] = ctx->t_mc_ms;
+        out[1] = (double)ctx->n_mc_launch;
+        out[2] = ctx->t_rebuild_ms;
+        out[3] = (double)ctx->n_rebuild_launch;
+        for (int k = 0; k < 5; ++k) out[4 + k] = (double)ctx->work[k];
     });
 }
 

@@ -114,6 +114,12 @@ struct ChainState {
     int status;
     int acc_win, tot_win;
     int mode;
+    long long clk[8];    // per-phase clock ticks (select, move, segments, pass A, radii, pass B, commit, total)
+    long long steps_by_kind[3];
+    // algorithmic work census: cross pairs (HCT change), in-cutoff pair
+    // evaluations (old + new), SASA row entries folded, moved atoms,
+    // boundary dihedral records
+    long long n_cross, n_pair_eval, n_sasa, n_moved, n_dih;
 };
 
 // Batch of chains of one system (same topology and parameters, own state).
@@ -148,6 +154,7 @@ struct ChainBatch {
     uint32_t* words = nullptr;  // nch * wcap mt19937 output words
     ChainState* st = nullptr;   // nch
     int* list = nullptr;        // chain ids of a batched launch
+    int4* seg = nullptr;        // nch * n * 4 cross-pair row segments (MC scratch)
     int trace_cap = 0;
     int* tr_kind = nullptr;
     int* tr_bond = nullptr;
@@ -178,6 +185,11 @@ struct alma_ctx {
     std::vector<double> h_pos;       // host copy of the current coordinates
     std::vector<double> h_q, h_r, h_e;
     bool have_eval = false;
+    // HIP-event timing of the last MC run: device MC kernel, list rebuilds
+    double t_mc_ms = 0.0, t_rebuild_ms = 0.0;
+    long long n_mc_launch = 0, n_rebuild_launch = 0;
+    long long work[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // MC trace of the last run
     std::vector<int> tr_kind, tr_bond, tr_acc;
     std::vector<double> tr_delta, tr_dE, tr_parts;

@@ -32,6 +32,8 @@ constexpr int kMcWaves = kMcBlock / kWave;
 constexpr int kSortBlock = 64;
 constexpr double kSasaSkin = 4.0;    // A beyond contact kept in the SASA sub-row
 constexpr double kSasaTrial = 1.5;   // max trial displacement served by it
+constexpr int kLdsRuns = 16;         // moved-set runs cached in LDS
+constexpr int kSegRuns = 3;          // runs handled by the segment path
 
 __device__ __forceinline__ double* P4(const ChainBatch& b, double* base, int c) {
     return base + (size_t)c * 4 * b.n;
@@ -568,6 +570,8 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     __shared__ Move mv;
     __shared__ double red[4 * kMcWaves];
     __shared__ int far_move;
+    __shared__ int2 lruns[2][kLdsRuns];
+    __shared__ int nseg;
     __shared__ Words W;
 
     if (tid == 0) {
@@ -578,7 +582,16 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     }
     __syncthreads();
 
+    long long t_prev = 0;
+    auto tick = [&](int ph) {
+        if (tid == 0) {
+            long long now = wall_clock64();
+            cs->clk[ph] += now - t_prev;
+            t_prev = now;
+        }
+    };
     for (;;) {
+        if (tid == 0) t_prev = wall_clock64();
         // ---- move selection (lane 0 of wave 0; RNG order of the reference)
         if (tid == 0) {
             mv.stop = 0;
@@ -670,11 +683,20 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
         __syncthreads();
         if (mv.stop) break;
 
-        const Runs r1{T.side_run + T.side_run_off[mv.bond >= 0 ? mv.bond : 0],
-                      mv.bond >= 0 ? T.side_run_off[mv.bond + 1] - T.side_run_off[mv.bond] : 0};
-        const Runs r2{T.side_run + T.side_run_off[mv.bond2 >= 0 ? mv.bond2 : 0],
-                      (mv.kind == 2 && mv.bond2 >= 0) ? T.side_run_off[mv.bond2 + 1] - T.side_run_off[mv.bond2]
-                                                      : 0};
+        // index runs of the moved groups, cached in LDS when short
+        const int nr1 = mv.bond >= 0 ? T.side_run_off[mv.bond + 1] - T.side_run_off[mv.bond] : 0;
+        const int nr2 = mv.bond2 >= 0 ? T.side_run_off[mv.bond2 + 1] - T.side_run_off[mv.bond2] : 0;
+        const int2* g1 = T.side_run + T.side_run_off[mv.bond >= 0 ? mv.bond : 0];
+        const int2* g2 = T.side_run + T.side_run_off[mv.bond2 >= 0 ? mv.bond2 : 0];
+        if (tid < kLdsRuns) {
+            if (tid < nr1) lruns[0][tid] = g1[tid];
+            if (tid < nr2) lruns[1][tid] = g2[tid];
+        }
+        __syncthreads();
+        tick(0);
+        const Runs r1{nr1 <= kLdsRuns ? lruns[0] : g1, nr1};
+        const Runs r2{nr2 <= kLdsRuns ? lruns[1] : g2, mv.kind == 2 ? nr2 : 0};
+        const Runs rpsi{nr2 <= kLdsRuns ? lruns[1] : g2, mv.kind == 1 ? nr2 : 0};
         const int n1 = mv.valid ? r1.count() : 0;
         const int n2 = (mv.valid && mv.kind == 2) ? r2.count() : 0;
 
@@ -699,8 +721,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             }
             __syncthreads();
             if (mv.valid) {
-                const Runs rp{T.side_run + T.side_run_off[mv.bond2],
-                              T.side_run_off[mv.bond2 + 1] - T.side_run_off[mv.bond2]};
+                const Runs& rp = rpsi;
                 int np = rp.count();
                 for (int k = tid; k < np; k += kMcBlock) {
                     int i = rp.at(k);
@@ -711,6 +732,8 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             __syncthreads();
         }
 
+        tick(1);
+        long long cnt_cross = 0, cnt_eval = 0, cnt_sasa = 0, cnt_dih = 0;
         if (mv.valid) {
             // enumeration set: the moved set or its complement, whichever is
             // smaller (mode 0 / 1); both groups of a concerted move (mode 2)
@@ -723,8 +746,54 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 return k < n1 ? r1.at(k) : r2.at(k - n1);
             };
 
+            // Fast path (modes 0/1, at most kSegRuns runs): every thread finds
+            // the partner ranges of its enumeration atoms in their j-sorted
+            // rows, non-empty ranges are compacted into a segment list, then
+            // wavefronts stream the segments with all 64 lanes.
+            const bool seg_path = mode != 2 && r1.n <= kSegRuns;
+            int4* segs = b.seg + R1(b, c) * (kSegRuns + 1);
+            if (seg_path) {
+                if (tid == 0) nseg = 0;
+                __syncthreads();
+                for (int k = tid; k < nT; k += kMcBlock) {
+                    int i = elem(k);
+                    int gi = mode == 0 ? 1 : 0;
+                    const uint32_t* row = nlf + (size_t)i * b.cap_f;
+                    int cnt = min(nlf_cnt[i], b.cap_f);
+                    int bnd[2 * kSegRuns];
+                    for (int q = 0; q < r1.n; ++q) {
+                        bnd[2 * q] = lower_bound_row(row, q ? bnd[2 * q - 1] : 0, cnt, r1.r[q].x);
+                        bnd[2 * q + 1] = lower_bound_row(row, bnd[2 * q], cnt, r1.r[q].y + 1);
+                    }
+                    for (int q = 0; q <= r1.n; ++q) {
+                        int p0, p1;
+                        if (mode == 0) {
+                            p0 = q == 0 ? 0 : bnd[2 * q - 1];
+                            p1 = q == r1.n ? cnt : bnd[2 * q];
+                        } else {
+                            if (q == r1.n) break;
+                            p0 = bnd[2 * q];
+                            p1 = bnd[2 * q + 1];
+                        }
+                        if (p1 > p0) segs[atomicAdd(&nseg, 1)] = make_int4(i, p0, p1, gi);
+                    }
+                }
+                __syncthreads();
+            }
+            tick(2);
+
             // visit every cross pair (i from the enumeration set, j partner)
             auto for_cross = [&](auto&& fn) {
+                if (seg_path) {
+                    const int ns = nseg;
+                    for (int q = wave; q < ns; q += kMcWaves) {
+                        int4 g = segs[q];
+                        const uint32_t* row = nlf + (size_t)g.x * b.cap_f;
+                        for (int p = g.y + lane; p < g.z; p += kWave) fn(g.x, g.w, row[p], 1 - g.w, true);
+                        fn(g.x, g.w, 0u, 0, false);
+                    }
+                    return;
+                }
                 for (int k = wave; k < nT; k += kMcWaves) {
                     int i = elem(k);
                     int gi = mode == 0 ? 1 : mode == 1 ? 0 : group(i);
@@ -779,6 +848,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                         return;
                     }
                     int j = (int)(e & kIdxMask);
+                    cnt_cross += 1;
                     const double* pio = pos + 4 * i;
                     const double* pjo = pos + 4 * j;
                     const double* pin = gi ? posn + 4 * i : pio;
@@ -796,6 +866,8 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     }
                 });
             }
+            __syncthreads();
+            tick(3);
             if (mv.kind == 1) {
                 // crankshaft: atoms of the phi side outside the psi side turn
                 // once, the psi side twice, so pairs between the two parts
@@ -803,8 +875,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 // atoms are in_side) but update_born and the full SASA
                 // recomputation see them: feed them to the Born and SASA
                 // bookkeeping only.
-                const Runs rp{T.side_run + T.side_run_off[mv.bond2],
-                              T.side_run_off[mv.bond2 + 1] - T.side_run_off[mv.bond2]};
+                const Runs& rp = rpsi;
                 for (int k = wave; k < n1; k += kMcWaves) {
                     int i = r1.at(k);
                     if (rp.has(i)) continue;
@@ -858,6 +929,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 if (far) far_move = 1;
             }
             __syncthreads();
+            tick(4);
             // ---- pass B: cross-pair energy change, SASA, dihedral, SS
             double acc = 0.0, acc_sa = 0.0, acc_dh = 0.0, acc_ss = 0.0;
             for_cross([&](int i, int gi, uint32_t e, int gj, bool is_pair) {
@@ -876,6 +948,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 ni.w = (float)(gi ? anew[i] : a[i]);
                 nj.w = (float)(gj ? anew[j] : a[j]);
                 bool in_o = r2o <= kPairCut2, in_n = r2n <= kPairCut2;
+                cnt_eval += (int)in_o + (int)in_n;
                 if (in_o && in_n)
                     acc += pair_delta(r2o, r2n, oi, oj, ni, nj, Ri, Rj, ei, ej);
                 else if (in_o)
@@ -902,6 +975,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     m = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
                     row = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
                 }
+                cnt_sasa += m + min(b.nls_cnt[R1(b, c) + i], b.cap_s);
                 for (int k = 0; k < m; ++k) {
                     int j = (int)(row[k] & kIdxMask);
                     const double* pj = group(j) ? posn + 4 * j : pos + 4 * j;
@@ -920,6 +994,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     bool any_m = false, any_f = false;
                     for (int m = 0; m < 4; ++m) (group(q4[m]) ? any_m : any_f) = true;
                     if (!any_m || !any_f) continue;
+                    cnt_dih += 1;
                     double po[16], pn[16];
                     for (int m = 0; m < 4; ++m)
                         for (int d = 0; d < 4; ++d) {
@@ -969,6 +1044,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 if (acc_ok) cs->cur_e += dE;
             }
             __syncthreads();
+            tick(5);
             // ---- commit or discard
             if (mv.accept) {
                 for (int k = tid; k < n1 + n2; k += kMcBlock) {
@@ -992,9 +1068,22 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             __syncthreads();
         }
 
+        {
+            double w0 = wave_sum((double)cnt_cross), w1 = wave_sum((double)cnt_eval);
+            double w2 = wave_sum((double)cnt_sasa), w3 = wave_sum((double)cnt_dih);
+            if (lane == 0 && (w0 + w1 + w2 + w3) > 0.0) {
+                atomicAdd((unsigned long long*)&cs->n_cross, (unsigned long long)w0);
+                atomicAdd((unsigned long long*)&cs->n_pair_eval, (unsigned long long)w1);
+                atomicAdd((unsigned long long*)&cs->n_sasa, (unsigned long long)w2);
+                atomicAdd((unsigned long long*)&cs->n_dih, (unsigned long long)w3);
+            }
+        }
+        if (tid == 0 && mv.valid) cs->n_moved += n1 + n2;
+        tick(6);
         // ---- bookkeeping (tid 0)
         if (tid == 0) {
             int step = cs->steps_done;
+            if (mv.kind >= 0) cs->steps_by_kind[mv.kind] += 1;
             if (trace && step < b.trace_cap) {
                 b.tr_kind[step] = mv.kind;
                 b.tr_bond[step] = mv.kind == 1 || mv.kind == 2 ? -1 : mv.bond;
