@@ -319,6 +319,7 @@ void free_chains(ChainBatch& b) {
     dev_free(b.st);
     dev_free(b.list);
     dev_free(b.seg);
+    dev_free(b.aff);
     dev_free(b.tr_kind);
     dev_free(b.tr_bond);
     dev_free(b.tr_delta);
@@ -366,6 +367,7 @@ void alloc_chains(ChainBatch& b, const DevSys& s, int nch, int trace_steps) {
     dev_alloc(&b.st, nch);
     dev_alloc(&b.list, nch);
     dev_alloc(&b.seg, N * 4);
+    dev_alloc(&b.aff, N);
     HIP_TRY(hipMemset(b.dhalf, 0, sizeof(double) * N));
     HIP_TRY(hipMemset(b.flag, 0, sizeof(int) * N));
     if (trace_steps > 0) {

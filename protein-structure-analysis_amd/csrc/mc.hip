@@ -34,6 +34,7 @@ constexpr double kSasaSkin = 4.0;    // A beyond contact kept in the SASA sub-ro
 constexpr double kSasaTrial = 1.5;   // max trial displacement served by it
 constexpr int kLdsRuns = 16;         // moved-set runs cached in LDS
 constexpr int kSegRuns = 3;          // runs handled by the segment path
+constexpr int kLdsAtoms = 11264;     // systems up to this size keep per-step atom state in LDS
 
 __device__ __forceinline__ double* P4(const ChainBatch& b, double* base, int c) {
     return base + (size_t)c * 4 * b.n;
@@ -561,9 +562,13 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     double* a = b.a + R1(b, c);
     double* anew = b.anew + R1(b, c);
     double* half = b.half + R1(b, c);
-    double* dhalf = b.dhalf + R1(b, c);
-    int* flag = b.flag + R1(b, c);
+    // per-step Born deltas and SASA marks: LDS when the system fits
+    extern __shared__ double dsm[];
+    const bool lds_atoms = n <= kLdsAtoms;
+    double* dhalf = lds_atoms ? dsm : b.dhalf + R1(b, c);
+    int* flag = lds_atoms ? (int*)(dsm + n) : b.flag + R1(b, c);
     const uint32_t* nlf = b.nlf + R1(b, c) * (size_t)b.cap_f;
+    int* aff = b.aff + R1(b, c);
     const int* nlf_cnt = b.nlf_cnt + R1(b, c);
     const bool trace = b.trace_cap > 0 && c == 0;
 
@@ -571,9 +576,14 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     __shared__ double red[4 * kMcWaves];
     __shared__ int far_move;
     __shared__ int2 lruns[2][kLdsRuns];
-    __shared__ int nseg;
+    __shared__ int nseg, naff;
     __shared__ Words W;
 
+    if (lds_atoms)
+        for (int i = tid; i < n; i += kMcBlock) {
+            dhalf[i] = 0.0;
+            flag[i] = 0;
+        }
     if (tid == 0) {
         W.w = b.words + (size_t)c * b.wcap;
         W.cur = cs->cursor;
@@ -834,17 +844,19 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             // lower index), marks atoms whose SASA fold changes
             {
                 double own = 0.0;
-                bool own_hit = false;
+                bool own_hit = false, own_near = false;
                 for_cross([&](int i, int gi, uint32_t e, int gj, bool is_pair) {
                     if (!is_pair) {   // end of atom i: flush its own accumulator
                         double v = wave_sum(own);
                         bool any = __ballot(own_hit) != 0ull;
+                        bool near = __ballot(own_near) != 0ull;
                         if (lane == 0 && any) {
                             atomicAdd(&dhalf[i], v);
-                            flag[i] = 1;
+                            atomicOr(&flag[i], near ? 3 : 1);
                         }
                         own = 0.0;
                         own_hit = false;
+                        own_near = false;
                         return;
                     }
                     int j = (int)(e & kIdxMask);
@@ -857,12 +869,16 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     int lo = min(i, j), hi = max(i, j);
                     float rl = s.prm[lo].y, rh = s.prm[hi].y;
                     double dlt = (double)hct_f(sqrtf(r2n), r2n, rl, rh) - (double)hct_f(sqrtf(r2o), r2o, rl, rh);
+                    // SASA contact range (with margin) in either geometry
+                    float dc = rl + rh + (float)(2.0 * kProbe) + 0.01f;
+                    bool near = fminf(r2o, r2n) < dc * dc;
                     if (lo == i) {
                         own += dlt;
                         own_hit = true;
+                        own_near |= near;
                     } else {
                         atomicAdd(&dhalf[j], dlt);
-                        flag[j] = 1;
+                        atomicOr(&flag[j], near ? 3 : 1);
                     }
                 });
             }
@@ -882,7 +898,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     const uint32_t* row = nlf + (size_t)i * b.cap_f;
                     int cnt = min(nlf_cnt[i], b.cap_f);
                     double own = 0.0;
-                    bool own_hit = false;
+                    bool own_hit = false, own_near = false;
                     for (int q = 0; q < rp.n; ++q) {
                         int p0 = lower_bound_row(row, 0, cnt, rp.r[q].x);
                         int p1 = lower_bound_row(row, p0, cnt, rp.r[q].y + 1);
@@ -894,20 +910,24 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                             float rl = s.prm[lo].y, rh = s.prm[hi].y;
                             double dlt = (double)hct_f(sqrtf(r2n), r2n, rl, rh) -
                                          (double)hct_f(sqrtf(r2o), r2o, rl, rh);
+                            float dc = rl + rh + (float)(2.0 * kProbe) + 0.01f;
+                            bool near = fminf(r2o, r2n) < dc * dc;
                             if (lo == i) {
                                 own += dlt;
                                 own_hit = true;
+                                own_near |= near;
                             } else {
                                 atomicAdd(&dhalf[j], dlt);
-                                flag[j] = 1;
+                                atomicOr(&flag[j], near ? 3 : 1);
                             }
                         }
                     }
                     double v = wave_sum(own);
                     bool any = __ballot(own_hit) != 0ull;
+                    bool nr = __ballot(own_near) != 0ull;
                     if (lane == 0 && any) {
                         atomicAdd(&dhalf[i], v);
-                        flag[i] = 1;
+                        atomicOr(&flag[i], nr ? 3 : 1);
                     }
                 }
             }
@@ -956,37 +976,72 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 else if (in_n)
                     acc += pair_energy(r2n, ni, nj, Ri, Rj, ei, ej);
             });
-            for (int i = tid; i < n; i += kMcBlock) {
-                if (!flag[i]) continue;
-                double pi_n[3];
-                const double* pi = pos + 4 * i;
-                bool gi = group(i) != 0;
-                // new fold: moved atoms at trial coordinates
-                double ri = (double)s.prm[i].y + kProbe;
-                double sa = 4.0 * M_PI * ri * ri;
-                const double k2 = M_PI * ri;
-                for (int d = 0; d < 3; ++d) pi_n[d] = gi ? posn[4 * i + d] : pi[d];
-                int m;
-                const uint32_t* row;
-                if (far_move) {   // trial move beyond the sub-row's reach: whole half row
-                    m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
-                    row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
-                } else {
-                    m = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
-                    row = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
+            // SASA: compact the marked atoms, then one wavefront per atom
+            // gathers 64 row entries at a time, ballots the contacts (old and
+            // new geometry) and folds them in row order from shuffled caps
+            if (tid == 0) naff = 0;
+            __syncthreads();
+            for (int i = tid; i < n; i += kMcBlock)
+                if (flag[i] & 2) aff[atomicAdd(&naff, 1)] = i;
+            __syncthreads();
+            {
+                const int na = naff;
+                for (int q = wave; q < na; q += kMcWaves) {
+                    const int i = aff[q];
+                    const bool gi = group(i) != 0;
+                    const double* pio = pos + 4 * i;
+                    const double* pin = gi ? posn + 4 * i : pio;
+                    const double ri = (double)s.prm[i].y + kProbe;
+                    const double k2 = M_PI * ri;
+                    double sa_o = 4.0 * M_PI * ri * ri, sa_n = sa_o;
+                    int m;
+                    const uint32_t* row;
+                    if (far_move) {   // trial move beyond the sub-row's reach: whole half row
+                        m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
+                        row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+                    } else {
+                        m = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
+                        row = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
+                    }
+                    cnt_sasa += lane == 0 ? 2 * m : 0;
+                    for (int base = 0; base < m; base += kWave) {
+                        const int k = base + lane;
+                        double cap_o = 0.0, cap_n = 0.0;
+                        bool co = false, cn = false;
+                        if (k < m) {
+                            const int j = (int)(row[k] & kIdxMask);
+                            const double dc = ri + (double)s.prm[j].y + kProbe;
+                            const double* pjo = pos + 4 * j;
+                            const double* pjn = group(j) ? posn + 4 * j : pjo;
+                            double r2 = dist2_exact(pio, pjo);
+                            if (r2 < dc * dc) {
+                                double r = (double)sqrtf((float)r2);
+                                co = r < dc;
+                                cap_o = k2 * (dc - r);
+                            }
+                            r2 = dist2_exact(pin, pjn);
+                            if (r2 < dc * dc) {
+                                double r = (double)sqrtf((float)r2);
+                                cn = r < dc;
+                                cap_n = k2 * (dc - r);
+                            }
+                        }
+                        uint64_t mo = __ballot(co), mn = __ballot(cn);
+                        while (mo) {
+                            int bit = __ffsll((unsigned long long)mo) - 1;
+                            double cp = __shfl(cap_o, bit, kWave);
+                            sa_o -= fmin(sa_o * 0.85, cp);
+                            mo &= mo - 1ull;
+                        }
+                        while (mn) {
+                            int bit = __ffsll((unsigned long long)mn) - 1;
+                            double cp = __shfl(cap_n, bit, kWave);
+                            sa_n -= fmin(sa_n * 0.85, cp);
+                            mn &= mn - 1ull;
+                        }
+                    }
+                    if (lane == 0) acc_sa += kGammaSA * (fmax(0.0, sa_n) - fmax(0.0, sa_o));
                 }
-                cnt_sasa += m + min(b.nls_cnt[R1(b, c) + i], b.cap_s);
-                for (int k = 0; k < m; ++k) {
-                    int j = (int)(row[k] & kIdxMask);
-                    const double* pj = group(j) ? posn + 4 * j : pos + 4 * j;
-                    double dc = ri + (double)s.prm[j].y + kProbe;
-                    double r2 = dist2_exact(pi_n, pj);
-                    if (r2 >= dc * dc) continue;
-                    double r = (double)sqrtf((float)r2);
-                    if (r >= dc) continue;
-                    sa -= fmin(sa * 0.85, k2 * (dc - r));
-                }
-                acc_sa += kGammaSA * fmax(0.0, sa) - sasa_fold(b, s, c, i, pi, pos);
             }
             if (s.has_topo) {
                 for (int k = tid; k < T.n_dih; k += kMcBlock) {
@@ -1148,7 +1203,15 @@ hipError_t launch_chain_rebuild(ChainBatch& b, const DevSys& s, int L, int count
 
 hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int L, hipStream_t st) {
     if (L <= 0) return hipSuccess;
-    k_mc<<<L, kMcBlock, 0, st>>>(b, s);
+    size_t lds = b.n <= kLdsAtoms ? (size_t)b.n * (sizeof(double) + sizeof(int)) : 0;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_mc, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(kLdsAtoms * (sizeof(double) + sizeof(int))));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    k_mc<<<L, kMcBlock, lds, st>>>(b, s);
     return hipGetLastError();
 }
 

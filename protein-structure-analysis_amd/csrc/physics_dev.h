@@ -26,6 +26,14 @@ __device__ __forceinline__ float wave_sumf(float v) {
     return v;
 }
 
+// Hardware transcendental / reciprocal approximations (about 1 ulp), used in
+// the per-pair FP32 arithmetic; every per-term error they add stays near
+// 1e-7 relative, far inside the 1e-5 parity budget.
+__device__ __forceinline__ float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_f(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float exp_f(float x) { return __expf(x); }
+__device__ __forceinline__ float log_f(float x) { return __logf(x); }
+
 // Reference FP64 squared distance, no contraction into FMA (the reference's
 // d^2 < 196 and d^2 > 144 decisions must be reproduced exactly).
 __device__ __forceinline__ double dist2_exact(const double* pi, const double* pj) {
@@ -53,7 +61,8 @@ __device__ __forceinline__ bool in_sorted(const int* __restrict__ v, int b, int 
 __device__ __forceinline__ float hct_f(float r, float r2, float ri, float rj) {
     float L = fmaxf(fabsf(r - rj), ri), U = r + rj;
     if (ri >= U) return 0.0f;
-    return 1.0f / L - 1.0f / U + (r2 - rj * rj + ri * ri) / (2.0f * r * ri * ri) * logf(L / U) * 0.5f / r;
+    float iL = rcp_f(L), iU = rcp_f(U), ir = rcp_f(r);
+    return iL - iU + (r2 - rj * rj + ri * ri) * (0.25f * ir * ir) * rcp_f(ri * ri) * log_f(L * iU);
 }
 
 // ── fused pair energy ──────────────────────────────────────────────────────
@@ -70,12 +79,12 @@ __device__ __forceinline__ PairTerms pair_terms(double r2d, float4 pi, float4 pj
                                                 double ei, double ej) {
     PairTerms t;
     float r2 = (float)r2d;
-    float r = sqrtf(r2);
+    float r = sqrt_f(r2);
     float qp = pi.x * pj.x;
-    t.dh = (double)(((float)(kCoulomb / kEpsWater) * qp / r) * expf(-(float)kKappa * r));
+    t.dh = (double)(((float)(kCoulomb / kEpsWater) * qp * rcp_f(r)) * exp_f(-(float)kKappa * r));
     float D = pi.w * pj.w;
-    float fgb = sqrtf(r2 + D * expf(-r2 / (4.0f * D)));
-    t.gb = (double)((float)kGbCoef * qp / fgb);
+    float fgb = sqrt_f(r2 + D * exp_f(-r2 * rcp_f(4.0f * D)));
+    t.gb = (double)((float)kGbCoef * qp * rcp_f(fgb));
     double sig = Ri + Rj;
     double s2d = sig * sig / r2d;
     t.hc = 0.0;
@@ -90,12 +99,42 @@ __device__ __forceinline__ PairTerms pair_terms(double r2d, float4 pi, float4 pj
             t.hc = fmin(kHardScale * x * x, kHardCap);
         }
     } else {
-        float eps = sqrtf(pi.z * pj.z);
+        float eps = sqrt_f(pi.z * pj.z);
         float s2 = (float)s2d;
         float s6 = s2 * s2 * s2;
         t.lj = (double)(4.0f * eps * (s6 * s6 - s6));
     }
     return t;
+}
+
+// LJ + hard-core part of pair_terms alone
+__device__ __forceinline__ double lj_hc(double r2d, float4 pi, float4 pj, double Ri, double Rj, double ei,
+                                        double ej) {
+    const float sigf = pi.y + pj.y;
+    const float s2f = sigf * sigf * rcp_f((float)r2d);
+    if (s2f < 0.999f) {   // outside sigma: FP32 (|LJ| <= epsilon here)
+        float eps = sqrt_f(pi.z * pj.z);
+        float s6 = s2f * s2f * s2f;
+        return (double)(4.0f * eps * (s6 * s6 - s6));
+    }
+    double sig = Ri + Rj;
+    double s2d = sig * sig / r2d;
+    if (s2d > 1.0) {
+        double s6 = s2d * s2d * s2d;
+        double v = 4.0 * sqrt(ei * ej) * (s6 * s6 - s6);
+        double rc = sig * kHardFrac;
+        if (r2d < rc * rc) {
+            double c2 = rc * rc / r2d;
+            double c6 = c2 * c2 * c2;
+            double x = c6 * c6 - 1.0;
+            v += fmin(kHardScale * x * x, kHardCap);
+        }
+        return v;
+    }
+    float eps = sqrt_f(pi.z * pj.z);
+    float s2 = (float)s2d;
+    float s6 = s2 * s2 * s2;
+    return (double)(4.0f * eps * (s6 * s6 - s6));
 }
 
 __device__ __forceinline__ double pair_energy(double r2d, float4 pi, float4 pj, double Ri, double Rj,
@@ -116,22 +155,22 @@ __device__ __forceinline__ double pair_delta(double r2o, double r2n, float4 po_i
                                              float4 pn_i, float4 pn_j, double Ri, double Rj, double ei,
                                              double ej) {
     float ro2 = (float)r2o, rn2 = (float)r2n;
-    float ro = sqrtf(ro2), rn = sqrtf(rn2);
+    float ro = sqrt_f(ro2), rn = sqrt_f(rn2);
     float d2 = (float)(r2n - r2o);
-    float dr = d2 / (ro + rn);
+    float dr = d2 * rcp_f(ro + rn);
     float qp = po_i.x * po_j.x;
     // Debye-Hueckel
-    float dh = (float)(kCoulomb / kEpsWater) * qp * expf(-(float)kKappa * ro) *
-               (ro * expm1f(-(float)kKappa * dr) - dr) / (ro * rn);
-    // generalized Born
+    float dh = (float)(kCoulomb / kEpsWater) * qp * exp_f(-(float)kKappa * ro) *
+               (ro * expm1f(-(float)kKappa * dr) - dr) * rcp_f(ro * rn);
+    // generalized Born (the Born exponentials vanish beyond r^2 > 87 * 4D)
     float Do = po_i.w * po_j.w, Dn = pn_i.w * pn_j.w;
-    float go = Do * expf(-ro2 / (4.0f * Do)), gn = Dn * expf(-rn2 / (4.0f * Dn));
-    float fo = sqrtf(ro2 + go), fn = sqrtf(rn2 + gn);
-    float gb = (float)kGbCoef * qp * ((-d2) + (go - gn)) / (fo * fn * (fo + fn));
+    float xo = ro2 * rcp_f(4.0f * Do), xn = rn2 * rcp_f(4.0f * Dn);
+    float go = xo < 87.0f ? Do * exp_f(-xo) : 0.0f, gn = xn < 87.0f ? Dn * exp_f(-xn) : 0.0f;
+    float fo = sqrt_f(ro2 + go), fn = sqrt_f(rn2 + gn);
+    float gb = (float)kGbCoef * qp * ((-d2) + (go - gn)) * rcp_f(fo * fn * (fo + fn));
     // Lennard-Jones + hard core
-    PairTerms to = pair_terms(r2o, po_i, po_j, Ri, Rj, ei, ej);
-    PairTerms tn = pair_terms(r2n, pn_i, pn_j, Ri, Rj, ei, ej);
-    return (double)dh + (double)gb + (tn.lj - to.lj) + (tn.hc - to.hc);
+    return (double)dh + (double)gb + lj_hc(r2n, pn_i, pn_j, Ri, Rj, ei, ej) -
+           lj_hc(r2o, po_i, po_j, Ri, Rj, ei, ej);
 }
 
 // ── dihedral / disulfide (FP64) ─────────────────────────────────────────────
