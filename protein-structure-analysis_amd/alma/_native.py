@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -46,13 +47,23 @@ class McStats(C.Structure):
 
 
 _LIB = None
+_WITH_TORCH_RUNTIME = False
+
+
+def shares_torch_runtime() -> bool:
+    """True when libalma was loaded after torch, i.e. it is bound to the HIP
+    runtime torch ships (one runtime in the process).  Loaded before torch, it
+    binds to the system runtime, and a later torch import brings a second
+    runtime that does not see the device."""
+    return _LIB is not None and _WITH_TORCH_RUNTIME
 
 
 def lib():
-    global _LIB
+    global _LIB, _WITH_TORCH_RUNTIME
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libalma.so is not built; run __graft_entry__.build()")
+        _WITH_TORCH_RUNTIME = "torch" in sys.modules
         L = C.CDLL(LIB_PATH)
         P = C.c_void_p
         L.alma_create.argtypes = [C.c_int, C.POINTER(P)]

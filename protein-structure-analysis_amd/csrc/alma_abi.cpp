@@ -79,6 +79,14 @@ void free_topo(DevTopo& t) {
     t = DevTopo{};
 }
 
+void free_forces(alma_ctx* ctx) {
+    dev_free(ctx->f_grad);
+    dev_free(ctx->f_dEda);
+    dev_free(ctx->f_pre);
+    dev_free(ctx->f_dEdt);
+    ctx->f_n = -1;
+}
+
 void free_sys(DevSys& s) {
     dev_free(s.pos);
     dev_free(s.prm);
@@ -632,6 +640,7 @@ void alma_destroy(alma_ctx* ctx) {
     hipSetDevice(ctx->device);
     free_sys(ctx->sys);
     free_chains(ctx->chains);
+    free_forces(ctx);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -659,6 +668,7 @@ alma_status alma_set_system(alma_ctx* ctx, int32_t n, const double* xyz, const d
         HIP_TRY(hipSetDevice(ctx->device));
         free_sys(ctx->sys);
         free_chains(ctx->chains);
+        free_forces(ctx);
         DevSys& s = ctx->sys;
         s.n = n;
         ctx->have_eval = false;
@@ -790,10 +800,34 @@ alma_status alma_mc_timing(alma_ctx* ctx, double* out) {
 }
 
 alma_status alma_forces(alma_ctx* ctx, double* energy, double* dEdx, double* dEdtheta) {
-    (void)energy;
-    (void)dEdx;
-    (void)dEdtheta;
-    return guarded(ctx, [&] { throw Fail{ALMA_ERUNTIME, "device forces are not built into this library"}; });
+    return guarded(ctx, [&] {
+        if (!energy || !dEdx) throw Fail{ALMA_EINVAL, "energy and gradient buffers are required"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        DevSys& s = ctx->sys;
+        if (s.n == 0) {
+            *energy = 0.0;
+            return;
+        }
+        double terms[ALMA_NTERMS];
+        evaluate(ctx, terms);
+        *energy = terms[ALMA_NTERMS - 1];
+        if (ctx->f_n != s.n) {
+            free_forces(ctx);
+            dev_alloc(&ctx->f_grad, 3 * (size_t)s.n);
+            dev_alloc(&ctx->f_dEda, (size_t)s.n);
+            dev_alloc(&ctx->f_pre, 6 * ((size_t)s.n + 1));
+            dev_alloc(&ctx->f_dEdt, (size_t)std::max(s.topo.n_rot, 1));
+            ctx->f_n = s.n;
+        }
+        const bool want_t = dEdtheta && s.has_topo && s.topo.n_rot > 0;
+        HIP_TRY(launch_forces(s, ctx->f_grad, ctx->f_dEda, ctx->f_pre, want_t ? ctx->f_dEdt : nullptr,
+                              ctx->stream));
+        HIP_TRY(hipMemcpyAsync(dEdx, ctx->f_grad, sizeof(double) * 3 * s.n, hipMemcpyDeviceToHost, ctx->stream));
+        if (want_t)
+            HIP_TRY(hipMemcpyAsync(dEdtheta, ctx->f_dEdt, sizeof(double) * s.topo.n_rot, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    });
 }
 
 }  // extern "C"

@@ -172,6 +172,9 @@ hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int nlist, hipStream_
 hipError_t launch_build_nl(DevSys& s, hipStream_t st);
 hipError_t launch_born(DevSys& s, hipStream_t st);
 hipError_t launch_energy(DevSys& s, hipStream_t st);
+// gradient of the total energy at the positions of the last launch_energy:
+// G (3n), dEda (n scratch), pre (6(n+1) scratch), dEdt (n_rot, may be null)
+hipError_t launch_forces(DevSys& s, double* G, double* dEda, double* pre, double* dEdt, hipStream_t st);
 
 }  // namespace alma
 
@@ -191,6 +194,9 @@ struct alma_ctx {
     long long n_mc_launch = 0, n_rebuild_launch = 0;
     long long work[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // forces buffers (sized for sys.n / n_rot when allocated)
+    double *f_grad = nullptr, *f_dEda = nullptr, *f_pre = nullptr, *f_dEdt = nullptr;
+    int f_n = -1;
     // MC trace of the last run
     std::vector<int> tr_kind, tr_bond, tr_acc;
     std::vector<double> tr_delta, tr_dE, tr_parts;

@@ -2,6 +2,10 @@ import os
 import sys
 
 import pytest
+# torch first: importing it loads its bundled HIP runtime, and libalma loaded
+# afterwards binds to that same runtime (one HIP runtime per process), so
+# tests that hand torch device tensors to the library share one device view.
+import torch  # noqa: F401
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "protein-structure-analysis_amd")

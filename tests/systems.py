@@ -18,6 +18,16 @@ def two_chains_ss():
     return s
 
 
+_SS_SEQ = ["ALA", "CYS", "LEU", "GLY", "SER", "VAL", "LYS", "GLU", "ASP", "CYS", "THR", "ILE",
+           "ASN", "GLY", "PHE", "CYS", "ARG", "GLN", "TYR", "CYS", "PRO", "HIS", "MET", "TRP"]
+
+
+@functools.lru_cache(maxsize=None)
+def chain_ss():
+    """24-residue compact chain with two disulfides (SG moved to 2.04 A)."""
+    return synth._add_disulfides(synth.make_chain(len(_SS_SEQ), 21, seq=list(_SS_SEQ)), 2)
+
+
 def bridge():
     """tests/bridge_test.py:83-85 of the reference: three particles."""
     return {"xyz": np.array([[0.0, 0, 0], [5.0, 0, 0], [10.0, 0, 0]]),

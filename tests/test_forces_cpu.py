@@ -1,0 +1,95 @@
+"""The oracle's analytic gradient against central finite differences of the
+oracle's own total energy (FP64 throughout, so h = 1e-5 resolves it).
+
+The energy is piecewise smooth: pair terms stop at 12 A and the Born and SASA
+sums at the 14 A list radius.  A coordinate whose +-h step moves any pair
+across either radius is skipped (its difference quotient measures the jump,
+not the derivative).  Torsion gradients are checked the same way, rotating
+each bond's j-side by +-h about the bond axis."""
+import numpy as np
+import pytest
+
+import systems as S
+from oracle import oracle as O
+
+H = 1e-5
+
+
+def _energy(s, c, xyz):
+    d = dict(s)
+    d["xyz"] = xyz
+    return O.energy_terms(d, c)["TOTAL"]
+
+
+def _near_radius(xyz, i, step):
+    r = np.linalg.norm(xyz - xyz[i], axis=1)
+    r[i] = 0.0
+    return np.any(np.abs(r - 12.0) < 2 * step) or np.any(np.abs(r - 14.0) < 2 * step)
+
+
+def _rotate(xyz, side, o, u, th):
+    v = xyz[side] - o
+    c, s_ = np.cos(th), np.sin(th)
+    out = xyz.copy()
+    out[side] = o + v * c + np.cross(u, v) * s_ + np.outer(v @ u, u) * (1 - c)
+    return out
+
+
+@pytest.mark.parametrize("name,make", [("chain24", lambda: S.chain(24, 3)), ("ss", S.chain_ss)])
+def test_gradient_matches_finite_differences(name, make):
+    s = make()
+    pairs = [tuple(int(v) for v in p) for p in s["ss_pairs"]]
+    c = S.csr(S.topology(s, pairs))
+    assert len(c["ss_i"]) == len(pairs)
+    E, g, gt = O.forces(s, c)
+    assert abs(E - _energy(s, c, s["xyz"])) <= 1e-9 * abs(E)
+    rng = np.random.default_rng(0)
+    n = len(s["xyz"])
+    atoms = rng.choice(n, size=min(n, 40), replace=False)
+    if pairs:   # the disulfide atoms carry the SS term
+        atoms = np.concatenate([np.array(pairs).ravel(), atoms])
+    checked = 0
+    for i in atoms:
+        if _near_radius(s["xyz"], i, H):
+            continue
+        for d in range(3):
+            xp = s["xyz"].copy()
+            xm = s["xyz"].copy()
+            xp[i, d] += H
+            xm[i, d] -= H
+            fd = (_energy(s, c, xp) - _energy(s, c, xm)) / (2 * H)
+            assert abs(fd - g[i, d]) <= 1e-4 * max(1.0, abs(g[i, d])), (name, i, d, fd, g[i, d])
+            checked += 1
+    assert checked >= 60
+    # torsions: every tenth bond
+    x = s["xyz"]
+    for b in range(0, len(c["rb_i"]), 10):
+        side = c["side_idx"][c["side_off"][b]:c["side_off"][b + 1]]
+        o = x[c["rb_i"][b]]
+        u = x[c["rb_j"][b]] - o
+        u = u / np.linalg.norm(u)
+        lever = np.max(np.linalg.norm(x[side] - o, axis=1))
+        if any(_near_radius(x, k, H * lever) for k in side):
+            continue
+        fd = (_energy(s, c, _rotate(x, side, o, u, H)) - _energy(s, c, _rotate(x, side, o, u, -H))) / (2 * H)
+        assert abs(fd - gt[b]) <= 1e-4 * max(1.0, abs(gt[b])), (name, b, fd, gt[b])
+
+
+def test_gradient_closed_form_two_charges():
+    """Two atoms, no topology: E(r) = DH + GB(r; a, a) + LJ; the gradient is
+    +-E'(r) along the axis, and E'(r) is checked against a central difference
+    of E(r) along the separation."""
+    s = {"xyz": np.array([[0.0, 0, 0], [4.3, 0, 0]]), "charge": np.array([0.7, -0.4]),
+         "rstar": np.array([1.9, 1.7]), "eps": np.array([0.1, 0.15])}
+    E, g, _ = O.forces(s, None)
+    assert g.shape == (2, 3)
+    assert np.allclose(g[0], -g[1], atol=1e-12)
+    assert abs(g[0, 1]) < 1e-12 and abs(g[0, 2]) < 1e-12
+
+    def e_at(r):
+        d = dict(s)
+        d["xyz"] = np.array([[0.0, 0, 0], [r, 0, 0]])
+        return O.energy_terms(d)["TOTAL"]
+
+    dEdr = (e_at(4.3 + 1e-6) - e_at(4.3 - 1e-6)) / 2e-6
+    assert abs(g[1, 0] - dEdr) <= 1e-6 * max(1.0, abs(dEdr))
