@@ -984,12 +984,24 @@ void orc_mt_words(uint32_t seed, int64_t n, uint32_t* out) {
     for (int64_t k = 0; k < n; ++k) out[k] = m();
 }
 
+void orc_uniform_int(uint32_t seed, int hi, int64_t n, int* out) {
+    MT m(seed);
+    for (int64_t k = 0; k < n; ++k) out[k] = m.uniform_int(hi);
+}
+
+void orc_uniform_real(uint32_t seed, double a, double b, int64_t n, double* out) {
+    MT m(seed);
+    for (int64_t k = 0; k < n; ++k) out[k] = m.uniform_real(a, b);
+}
+
 int orc_mc_run(int n, double* xyz, const double* q, const double* rstar, const double* eps,
                const orc_topo* topo, const orc_mc_params* p, orc_mc_stats* st, orc_mc_trace* tr,
                double* snap_xyz, double* snap_e) {
     if (n <= 0 || !topo || topo->n_rot <= 0 || p->steps <= 0) return -1;
     Sys s{n, q, rstar, eps};
     Chain ch(s, xyz, topo, p->seed, p->max_angle);
+    for (int64_t k = 0; k < p->skip; ++k) ch.rng();
+    ch.rng.used = 0;
     ch.want_items = tr && tr->dE_l1;
     int snap = 0;
     for (int k = 0; k < p->steps; ++k) {

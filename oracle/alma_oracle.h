@@ -51,6 +51,10 @@ double orc_dihedral_angle(const double* a, const double* b, const double* c, con
 
 /* ---- RNG: std::mt19937 + libstdc++-11 distributions ---- */
 void orc_mt_words(uint32_t seed, int64_t n, uint32_t* out);
+/* n draws of uniform_int_distribution<int>(0, hi) / uniform_real_distribution
+ * <double>(a, b) from mt19937(seed), as the MC loop consumes them. */
+void orc_uniform_int(uint32_t seed, int hi, int64_t n, int* out);
+void orc_uniform_real(uint32_t seed, double a, double b, int64_t n, double* out);
 
 /* ---- MC ---- */
 enum { ORC_MODE_ENSEMBLE = 0,   /* generate_ensemble chain semantics      */
@@ -64,6 +68,8 @@ typedef struct {
     double temperature;     /* kT in kcal/mol                                  */
     double max_angle;       /* initial cur_max (rad)                           */
     uint32_t seed;          /* mt19937 seed                                    */
+    int64_t skip;           /* words discarded before the chain starts (a later
+                               candidate of the serial generate_ensemble loop) */
 } orc_mc_params;
 
 typedef struct {

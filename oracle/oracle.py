@@ -33,7 +33,7 @@ class Topo(C.Structure):
 class McParams(C.Structure):
     _fields_ = [("mode", C.c_int), ("steps", C.c_int), ("snapshot_every", C.c_int),
                 ("temperature", C.c_double), ("max_angle", C.c_double),
-                ("seed", C.c_uint32)]
+                ("seed", C.c_uint32), ("skip", C.c_int64)]
 
 
 class McStats(C.Structure):
@@ -62,6 +62,8 @@ def lib():
         L.orc_dihedral_angle.restype = C.c_double
         L.orc_dihedral_angle.argtypes = [_D, _D, _D, _D]
         L.orc_mt_words.argtypes = [C.c_uint32, C.c_int64, C.POINTER(C.c_uint32)]
+        L.orc_uniform_int.argtypes = [C.c_uint32, C.c_int, C.c_int64, C.POINTER(C.c_int)]
+        L.orc_uniform_real.argtypes = [C.c_uint32, C.c_double, C.c_double, C.c_int64, _D]
         L.orc_mc_run.restype = C.c_int
         L.orc_mc_run.argtypes = [C.c_int, _D, _D, _D, _D, C.POINTER(Topo), C.POINTER(McParams),
                                  C.POINTER(McStats), C.POINTER(McTrace), _D, _D]
@@ -160,12 +162,30 @@ def mt_words(seed, n):
     return out
 
 
+def uniform_int(seed, hi, n):
+    out = np.zeros(n, np.int32)
+    lib().orc_uniform_int(seed, hi, n, out.ctypes.data_as(C.POINTER(C.c_int)))
+    return out
+
+
+def uniform_real(seed, a, b, n):
+    out = np.zeros(n)
+    lib().orc_uniform_real(seed, a, b, n, _p(out, C.c_double))
+    return out
+
+
+def dihedral_angle(q):
+    """Reference dihedral of four points (4 x 3)."""
+    q = _f64(q).reshape(4, 3)
+    return lib().orc_dihedral_angle(*[_p(np.ascontiguousarray(q[k]), C.c_double) for k in range(4)])
+
+
 def mc_run(sysd, topo_csr, steps, temperature=0.6, max_angle=0.12, seed=5489,
-           mode=MODE_ENSEMBLE, snapshot_every=0, trace=False, l1=False):
+           mode=MODE_ENSEMBLE, snapshot_every=0, trace=False, l1=False, skip=0):
     xyz = _f64(sysd["xyz"]).copy()
     n = len(xyz)
     th = TopoHandle(topo_csr, n)
-    p = McParams(mode, steps, snapshot_every, temperature, max_angle, seed)
+    p = McParams(mode, steps, snapshot_every, temperature, max_angle, seed, skip)
     st = McStats()
     tr = None
     arrays = {}

@@ -1,0 +1,126 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every entry point include/alma.h declares, the drop-in module exposes the
+reference's Python surface (src/physics_engine.cpp:3692-3888), and its
+topology builder follows the reference's rules on small hand-made inputs."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "alma.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(alma_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from alma import _native
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 13
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_native.EXPORTED) <= set(names)
+
+
+def test_create_reports_missing_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    from alma import _native
+    h = ctypes.c_void_p()
+    st = _native.lib().alma_create(0, ctypes.byref(h))
+    assert st == _native.ALMA_ENODEV
+    assert not h.value
+    # null-context calls fail cleanly
+    assert _native.lib().alma_last_error(None) == b"null context"
+
+
+def test_module_surface():
+    import protein_physics as PP
+    for name in ("Particle", "BondKind", "RotBond", "BondTopology", "PhysicsEngine"):
+        assert hasattr(PP, name), name
+    for m in ("calculate_potential", "generate_ensemble", "run_landscape_trajectory",
+              "run_landscape_segment", "lowest_energy_structure", "num_threads", "device_name"):
+        assert hasattr(PP.PhysicsEngine, m), m
+    for m in ("build", "bonded", "j_side", "is_excluded", "identify_concerted_sidechain_pairs",
+              "add_disulfide"):
+        assert hasattr(PP.BondTopology, m), m
+    for f in ("adj", "bonds", "rot_bonds", "rot_bond_sides", "excl", "rot_bond_scale",
+              "disulfide_pairs", "concerted_pairs", "concerted_sidechain_pairs", "rb_atom_i",
+              "rb_atom_j", "rb_kind", "dih_a", "dih_b", "dih_c", "dih_d", "dih_term_offsets",
+              "dih_term_v2", "dih_term_n", "dih_term_gamma"):
+        assert hasattr(PP.BondTopology, f), f
+    assert int(PP.BondKind.BACKBONE_PHI) == 0 and int(PP.BondKind.FIXED) == 3
+    p = PP.Particle(1.0, 2.0, 3.0, -0.5)
+    assert (p.radius, p.epsilon, p.is_water) == (1.9, 0.1, False)
+
+
+def test_engine_without_device_raises():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    import protein_physics as PP
+    with pytest.raises(RuntimeError):
+        PP.PhysicsEngine().calculate_potential([PP.Particle(0, 0, 0, 1.0)])
+
+
+def _dipeptide():
+    names = ["N", "H1", "H2", "H3", "CA", "HA", "CB", "HB1", "HB2", "HB3", "C", "O",
+             "N", "H", "CA", "HA2", "HA3", "C", "O", "OXT"]
+    res = ["ALA"] * 12 + ["GLY"] * 8
+    idx = [1] * 12 + [2] * 8
+    return res, names, idx
+
+
+def test_topology_rules_on_a_dipeptide():
+    """Peptide bond C(r)-N(r+1), 1-2 and 1-3 exclusions, the phi/psi
+    rotatable bonds with their j-sides and the side-size scale."""
+    import protein_physics as PP
+    res, names, idx = _dipeptide()
+    t = PP.BondTopology()
+    t.build(res, names, idx)
+    at = {(r, n): k for k, (r, n) in enumerate(zip(idx, names))}
+    C1, N2, CA2, CA1 = at[(1, "C")], at[(2, "N")], at[(2, "CA")], at[(1, "CA")]
+    assert t.bonded(C1, N2)
+    assert not t.bonded(CA1, CA2)
+    # excl: 1-2 and 1-3 only
+    assert t.is_excluded(C1, N2) and t.is_excluded(CA1, N2)
+    assert not t.is_excluded(CA1, CA2)
+    ex = t.excl
+    for i, row in enumerate(ex):
+        assert list(row) == sorted(set(row))
+        for j in row:
+            two_hop = t.bonded(i, j) or any(t.bonded(i, k) and t.bonded(k, j) for k in range(len(names)))
+            assert two_hop
+    # every rotatable bond's side is the connected component of j without i
+    for (i, j), side in zip(zip(t.rb_atom_i, t.rb_atom_j), t.rot_bond_sides):
+        assert j in side and i not in side
+        scale = min(1.0, max(0.05, (10.0 / len(side)) ** 0.5))
+        k = list(zip(t.rb_atom_i, t.rb_atom_j)).index((i, j))
+        assert abs(t.rot_bond_scale[k] - scale) <= 1e-15
+    # disulfide insertion keeps excl sorted
+    t.add_disulfide(at[(1, "CB")], at[(2, "CA")])
+    row = list(t.excl[at[(1, "CB")]])
+    assert at[(2, "CA")] in row and row == sorted(row)
+    assert tuple(t.disulfide_pairs[0]) == (at[(1, "CB")], at[(2, "CA")])
+
+
+def test_synthetic_generator_is_seeded():
+    from alma import synth
+    a = synth.make_chain(12, 3)
+    b = synth.make_chain(12, 3)
+    assert np.array_equal(a["xyz"], b["xyz"])
+    assert a["atomnames"] == b["atomnames"]
+    assert len(a["xyz"]) == len(a["charge"]) == len(a["resnames"])
+    # bonded geometry: every N-CA distance near 1.46 A
+    n = [k for k, nm in enumerate(a["atomnames"]) if nm == "N"]
+    ca = [k for k, nm in enumerate(a["atomnames"]) if nm == "CA"]
+    d = np.linalg.norm(a["xyz"][n] - a["xyz"][ca], axis=1)
+    assert np.all(np.abs(d - 1.46) < 0.05)
