@@ -54,6 +54,7 @@ def main():
     ap.add_argument("--residues", type=int, default=300)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--force-reps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -112,6 +113,12 @@ def main():
     single = args.single_steps / (time.perf_counter() - t1)
     log("single chain: %.1f evals/s" % single)
 
+    # full evaluation (calculate_potential: list + Born + all terms) against
+    # the same evaluation plus the backward kernels (forces and torsion
+    # gradients), same coordinates, host round trip included in both
+    forces = _forces_rate(ctx, args.force_reps)
+    log("energy %.1f evals/s, forces %.1f evals/s" % (forces["energy_evals_per_s"], forces["force_evals_per_s"]))
+
     flops = sum(CENSUS[k] * work[k] for k in CENSUS)
     achieved = flops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
     roofline = {"bound": "mfma", "roof": "FP32 vector ALU peak (no MFMA: the pair sums are not a dense contraction)", "achieved": achieved,
@@ -145,10 +152,27 @@ def main():
                    "device": dev},
         "single_chain_evals_per_s": single,
         "accept_rate": acc / max(evals, 1),
+        "forces": forces,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _forces_rate(ctx, reps):
+    ctx.energy()
+    ctx.forces()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.energy()
+    te = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.forces()
+    tf = (time.perf_counter() - t0) / reps
+    return {"energy_evals_per_s": 1.0 / te, "force_evals_per_s": 1.0 / tf, "force_over_energy_cost": tf / te,
+            "note": "full calculate_potential evaluation vs the same plus dE/dx (Born chain rule) and "
+                    "dE/dtheta, one system, host round trip included"}
 
 
 def _csr(t):

@@ -2,6 +2,7 @@
 
   libalma.so            gfx950 HIP kernels + the C ABI (include/alma.h), hipcc
   protein_physics*.so   the drop-in pybind11 module over the C ABI, g++
+  protein_physics_cuda*.so  the same module under the reference GPU module's name
   oracle/liboracle.so   (+ oracle/_ref when the reference sources exist):
                         test-infrastructure builds, via oracle/Makefile
 
@@ -19,6 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libalma.so")
 EXT = sysconfig.get_config_var("EXT_SUFFIX")
 MODULE = os.path.join(PKG, "protein_physics" + EXT)
+MODULE_GPU_NAME = os.path.join(PKG, "protein_physics_cuda" + EXT)
 
 HIP_SOURCES = ["energy.hip", "mc.hip", "forces.hip", "alma_abi.cpp"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -43,12 +45,20 @@ def build_lib():
 def build_module():
     import pybind11
     inc = sysconfig.get_paths()["include"]
-    tmp = MODULE + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
-          "-I", inc, "-I", pybind11.get_include(), "-I", os.path.join(ROOT, "include"),
-          os.path.join(CSRC, "protein_physics.cpp"), os.path.join(CSRC, "topology.cpp"),
-          "-L", PKG, "-lalma", "-Wl,-rpath,$ORIGIN", "-o", tmp])
-    os.replace(tmp, MODULE)
+    base = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+            "-I", inc, "-I", pybind11.get_include(), "-I", os.path.join(ROOT, "include"),
+            os.path.join(CSRC, "protein_physics.cpp"), os.path.join(CSRC, "topology.cpp"),
+            "-L", PKG, "-lalma", "-Wl,-rpath,$ORIGIN"]
+    jobs = []
+    for out, name in ((MODULE, "protein_physics"), (MODULE_GPU_NAME, "protein_physics_cuda")):
+        cmd = base + ["-DALMA_PY_MODULE=" + name, "-o", out + ".tmp"]
+        jobs.append((out, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                                text=True)))
+    for out, cmd, pr in jobs:
+        log = pr.communicate()[0]
+        if pr.returncode != 0:
+            raise RuntimeError("build step failed: %s\n%s" % (" ".join(cmd), log[-6000:]))
+        os.replace(out + ".tmp", out)
 
 
 def build_oracle():

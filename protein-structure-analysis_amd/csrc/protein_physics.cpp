@@ -279,7 +279,14 @@ void check_mc_args(const Parts& init, const Csr& t, int a, int b, const char* wh
 
 using namespace alma_py;
 
-PYBIND11_MODULE(protein_physics, m) {
+// Built twice: as `protein_physics` (the CPU module's name, for callers that
+// import it directly) and as `protein_physics_cuda` (the name the reference
+// GUI probes for its GPU backend, python/gui_main.py:150-156).
+#ifndef ALMA_PY_MODULE
+#define ALMA_PY_MODULE protein_physics
+#endif
+
+PYBIND11_MODULE(ALMA_PY_MODULE, m) {
     m.doc() = "MI355X implicit-solvent engine (HIP neighbour list, HCT-GB, SASA, device MC)";
 
     py::class_<Particle>(m, "Particle", py::module_local())
