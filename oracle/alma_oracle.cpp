@@ -929,6 +929,105 @@ double gradient(const Sys& s, const double* x, const orc_topo* t, double* G) {
     return terms[ORC_TOTAL];
 }
 
+// ─────────────────────── MALA torsion moves + PT ────────────────────────────
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of
+// (hi, lo) = mulhilo(M, c) with the Weyl key schedule.
+void philox(const uint32_t c0[4], const uint32_t k0[2], uint32_t out[4]) {
+    uint32_t c[4] = {c0[0], c0[1], c0[2], c0[3]}, k[2] = {k0[0], k0[1]};
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c[1] ^ k[0], n2 = hi0 ^ c[3] ^ k[1];
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k[0] += 0x9E3779B9u;
+        k[1] += 0xBB67AE85u;
+    }
+    for (int q = 0; q < 4; ++q) out[q] = c[q];
+}
+
+constexpr double TWO_M32 = 1.0 / 4294967296.0;
+
+double torque(const double* x, const double* G, const orc_topo* t, int k) {
+    int bi = t->rb_i[k], bj = t->rb_j[k];
+    double o[3], u[3];
+    for (int d = 0; d < 3; ++d) {
+        o[d] = x[3 * bi + d];
+        u[d] = x[3 * bj + d] - o[d];
+    }
+    double l = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    if (l < 1e-10) return 0.0;
+    for (int d = 0; d < 3; ++d) u[d] /= l;
+    double acc = 0.0;
+    for (int m = t->side_off[k]; m < t->side_off[k + 1]; ++m) {
+        int i = t->side_idx[m];
+        double v[3] = {x[3 * i] - o[0], x[3 * i + 1] - o[1], x[3 * i + 2] - o[2]}, w[3];
+        cross3(v, &G[3 * i], w);
+        acc += u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+    }
+    return acc;
+}
+
+struct MalaSys {
+    Sys s;
+    const orc_topo* t;
+    double* x;
+    std::vector<double> G, Gn, xn;
+    double E = 0.0;
+    int64_t accepted = 0, moves = 0;
+};
+
+// one MALA step of one system at temperature kT
+void mala_step(MalaSys& m, uint32_t seed, uint32_t sys, int step, double kT, double h, int* tb, double* td,
+               double* tE, int* ta) {
+    const orc_topo* t = m.t;
+    const int n = m.s.n;
+    uint32_t ctr[4] = {(uint32_t)step, 0u, 0u, 0u}, key[2] = {seed, sys}, w[4];
+    philox(ctr, key, w);
+    int k = (int)(((uint64_t)w[0] * (uint64_t)t->n_rot) >> 32);
+    double xi = std::sqrt(-2.0 * std::log(((double)w[1] + 1.0) * TWO_M32)) * std::cos(2.0 * PI * ((double)w[2] * TWO_M32));
+    double u = ((double)w[3] + 0.5) * TWO_M32;
+    double beta = 1.0 / kT;
+    int bi = t->rb_i[k], bj = t->rb_j[k];
+    double o[3], ax[3];
+    for (int d = 0; d < 3; ++d) {
+        o[d] = m.x[3 * bi + d];
+        ax[d] = m.x[3 * bj + d] - o[d];
+    }
+    double l = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    double delta = 0.0, dE = 0.0;
+    bool acc = false;
+    if (l >= 1e-10 && t->side_off[k + 1] > t->side_off[k]) {
+        for (int d = 0; d < 3; ++d) ax[d] /= l;
+        double g = torque(m.x, m.G.data(), t, k);
+        double mu_f = -0.5 * h * h * beta * g;
+        delta = mu_f + h * xi;
+        m.xn.assign(m.x, m.x + 3 * (size_t)n);
+        double c = std::cos(delta), sn = std::sin(delta);
+        for (int q = t->side_off[k]; q < t->side_off[k + 1]; ++q) rodrigues(&m.xn[3 * t->side_idx[q]], o, ax, c, sn);
+        double En = gradient(m.s, m.xn.data(), t, m.Gn.data());
+        double gn = torque(m.xn.data(), m.Gn.data(), t, k);
+        double mu_r = -0.5 * h * h * beta * gn;
+        dE = En - m.E;
+        double lf = (delta - mu_f) * (delta - mu_f), lr = (-delta - mu_r) * (-delta - mu_r);
+        double log_alpha = -beta * dE - (lr - lf) / (2.0 * h * h);
+        ++m.moves;
+        if (std::log(u) < log_alpha) {
+            acc = true;
+            ++m.accepted;
+            std::copy(m.xn.begin(), m.xn.end(), m.x);
+            std::swap(m.G, m.Gn);
+            m.E = En;
+        }
+    }
+    if (tb) *tb = k;
+    if (td) *td = delta;
+    if (tE) *tE = dE;
+    if (ta) *ta = acc ? 1 : 0;
+}
+
 }  // namespace
 
 // ───────────────────────────────── C ABI ────────────────────────────────────
@@ -1125,6 +1224,81 @@ double orc_forces(int n, const double* xyz, const double* q, const double* rstar
         }
     }
     return E;
+}
+
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
+
+int orc_pt_run(int n_prot, const int* n_atoms, const double* q, const double* rstar, const double* eps,
+               const orc_topo* topos, int n_rep, const double* temps, double* xyz, const orc_pt_params* p,
+               orc_pt_stats* st, int64_t* swaps, int* tr_bond, double* tr_delta, double* tr_dE, int* tr_acc,
+               int* tr_slot) {
+    if (n_prot <= 0 || n_rep <= 0 || p->steps < 0 || !(p->mala_step > 0.0)) return -1;
+    const int S = n_prot * n_rep;
+    std::vector<MalaSys> sys(S);
+    std::vector<int> slot_sys(S);   // slot (p * n_rep + a) -> system
+    size_t poff = 0, xoff = 0;
+    for (int pr = 0; pr < n_prot; ++pr) {
+        if (topos[pr].n_rot <= 0) return -1;
+        for (int r = 0; r < n_rep; ++r) {
+            MalaSys& m = sys[pr * n_rep + r];
+            m.s = Sys{n_atoms[pr], q + poff, rstar + poff, eps + poff};
+            m.t = &topos[pr];
+            m.x = xyz + xoff;
+            m.G.assign(3 * (size_t)n_atoms[pr], 0.0);
+            m.Gn.assign(3 * (size_t)n_atoms[pr], 0.0);
+            xoff += 3 * (size_t)n_atoms[pr];
+            slot_sys[pr * n_rep + r] = pr * n_rep + r;
+        }
+        poff += n_atoms[pr];
+    }
+    std::vector<int> slot_of(S);
+    for (int k = 0; k < S; ++k) slot_of[k] = k % n_rep;
+#pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < S; ++k) sys[k].E = gradient(sys[k].s, sys[k].x, sys[k].t, sys[k].G.data());
+    if (swaps)
+        for (int k = 0; k < 2 * n_prot; ++k) swaps[k] = 0;
+    const int chunk = p->exchange_every > 0 ? p->exchange_every : p->steps;
+    for (int t0 = 0; t0 < p->steps; t0 += chunk) {
+        int t1 = std::min(p->steps, t0 + chunk);
+#pragma omp parallel for schedule(dynamic)
+        for (int k = 0; k < S; ++k) {
+            for (int step = t0; step < t1; ++step) {
+                size_t o = (size_t)k * p->steps + step;
+                if (tr_slot) tr_slot[o] = slot_of[k];
+                mala_step(sys[k], p->seed, (uint32_t)k, step, temps[slot_of[k]], p->mala_step,
+                          tr_bond ? tr_bond + o : nullptr, tr_delta ? tr_delta + o : nullptr,
+                          tr_dE ? tr_dE + o : nullptr, tr_acc ? tr_acc + o : nullptr);
+            }
+        }
+        if (p->exchange_every > 0 && t1 % p->exchange_every == 0) {
+            int e = t1 / p->exchange_every - 1;
+            for (int pr = 0; pr < n_prot; ++pr) {
+                for (int a = e % 2; a + 1 < n_rep; a += 2) {
+                    int sa = slot_sys[pr * n_rep + a], sb = slot_sys[pr * n_rep + a + 1];
+                    double x = (sys[sa].E - sys[sb].E) * (1.0 / temps[a] - 1.0 / temps[a + 1]);
+                    uint32_t ctr[4] = {(uint32_t)e, (uint32_t)a, 1u, 0u}, key[2] = {p->seed, 0x80000000u | (uint32_t)pr}, w[4];
+                    philox(ctr, key, w);
+                    double v = ((double)w[0] + 0.5) * TWO_M32;
+                    if (swaps) ++swaps[2 * pr];
+                    if (x >= 0.0 || v < std::exp(x)) {
+                        if (swaps) ++swaps[2 * pr + 1];
+                        slot_sys[pr * n_rep + a] = sb;
+                        slot_sys[pr * n_rep + a + 1] = sa;
+                        slot_of[sb] = a;
+                        slot_of[sa] = a + 1;
+                    }
+                }
+            }
+        }
+    }
+    for (int k = 0; k < S; ++k) {
+        st[k].e = sys[k].E;
+        st[k].accepted = sys[k].accepted;
+        st[k].moves = sys[k].moves;
+        st[k].slot = slot_of[k];
+        st[k].pad = 0;
+    }
+    return 0;
 }
 
 int orc_num_threads(void) {

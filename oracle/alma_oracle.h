@@ -118,6 +118,48 @@ double orc_torsion_delta(int n, const double* xyz, const double* q, const double
 double orc_forces(int n, const double* xyz, const double* q, const double* rstar,
                   const double* eps, const orc_topo* topo, double* dEdx, double* dEdtheta);
 
+/* ---- gradient-guided (MALA) torsion moves and parallel tempering ----
+ * No reference counterpart: the build's own sampler.  Each step of system s
+ * draws Philox4x32-10(key = {seed, s}, counter = {step, 0, 0, 0}) -> w0..w3:
+ *   bond  k  = (w0 * n_rot) >> 32
+ *   xi       = sqrt(-2 ln((w1 + 1) 2^-32)) cos(2 pi w2 2^-32)
+ *   u        = (w3 + 0.5) 2^-32
+ * delta = -(h^2/2) beta g_k + h xi with g_k = dE/dtheta_k at the current
+ * state; accept iff ln u < -beta dE + ln q(x'->x) - ln q(x->x').  E is the
+ * calculate_potential energy (fresh list every evaluation).
+ * Exchange round e of protein p (after every exchange_every steps) tries the
+ * slot pairs (o, o+1), (o+2, o+3), ... with o = e % 2 (python/gui_main.py:
+ * 1998-2012): swap iff x >= 0 or v < exp(x), x = (E_a - E_b)(1/T_a - 1/T_b),
+ * v = (w0 + 0.5) 2^-32 of Philox(key = {seed, 0x80000000 | p},
+ * counter = {e, a, 1, 0}).  A swap exchanges the two systems' slots. */
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+typedef struct {
+    int steps;             /* MALA steps per system                            */
+    int exchange_every;    /* steps between exchange rounds (0 = none)         */
+    double mala_step;      /* h, rad                                           */
+    uint32_t seed;
+} orc_pt_params;
+
+typedef struct {
+    double e;              /* energy of the final state                        */
+    int64_t accepted;
+    int64_t moves;
+    int32_t slot;          /* final temperature slot                           */
+    int32_t pad;
+} orc_pt_stats;
+
+/* n_prot proteins (concatenated q/rstar/eps, one topology each, local
+ * indices), n_rep replicas of each; system s = p * n_rep + r starts at slot
+ * r with temperature kT temps[r].  xyz holds all systems in that order and
+ * is updated in place.  swaps: 2 per protein (attempted, accepted).  Trace
+ * arrays (steps per system, system-major; tr_slot = slot during the step)
+ * may be NULL. */
+int orc_pt_run(int n_prot, const int* n_atoms, const double* q, const double* rstar,
+               const double* eps, const orc_topo* topos, int n_rep, const double* temps,
+               double* xyz, const orc_pt_params* p, orc_pt_stats* st, int64_t* swaps,
+               int* tr_bond, double* tr_delta, double* tr_dE, int* tr_acc, int* tr_slot);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -62,6 +62,11 @@ def lib():
         L.orc_dihedral_angle.restype = C.c_double
         L.orc_dihedral_angle.argtypes = [_D, _D, _D, _D]
         L.orc_mt_words.argtypes = [C.c_uint32, C.c_int64, C.POINTER(C.c_uint32)]
+        L.orc_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_pt_run.restype = C.c_int
+        L.orc_pt_run.argtypes = [C.c_int, _I, _D, _D, _D, C.POINTER(Topo), C.c_int, _D, _D,
+                                 C.POINTER(PtParams), C.POINTER(PtStats), C.POINTER(C.c_int64),
+                                 _I, _D, _D, _I, _I]
         L.orc_uniform_int.argtypes = [C.c_uint32, C.c_int, C.c_int64, C.POINTER(C.c_int)]
         L.orc_uniform_real.argtypes = [C.c_uint32, C.c_double, C.c_double, C.c_int64, _D]
         L.orc_mc_run.restype = C.c_int
@@ -260,6 +265,70 @@ def forces(sysd, topo_csr):
                          _p(_f64(sysd["rstar"]), C.c_double), _p(_f64(sysd["eps"]), C.c_double),
                          th.ptr() if th else None, _p(g, C.c_double), _p(gt, C.c_double))
     return E, g, gt[:nrot]
+
+
+class PtParams(C.Structure):
+    _fields_ = [("steps", C.c_int), ("exchange_every", C.c_int), ("mala_step", C.c_double),
+                ("seed", C.c_uint32)]
+
+
+class PtStats(C.Structure):
+    _fields_ = [("e", C.c_double), ("accepted", C.c_int64), ("moves", C.c_int64),
+                ("slot", C.c_int32), ("pad", C.c_int32)]
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox(c, k, out)
+    return list(out)
+
+
+def pt_run(proteins, topo_csrs, n_rep, temps, xyz, steps, exchange_every=0, mala_step=0.02, seed=1,
+           trace=False):
+    """proteins: list of system dicts (q/rstar/eps used); xyz: list of n_prot * n_rep
+    coordinate arrays (system s = p * n_rep + r).  Returns final coordinates,
+    per-system stats, swap counts and (optionally) the per-step trace."""
+    n_prot = len(proteins)
+    n_atoms = np.array([len(pr["charge"]) for pr in proteins], np.int32)
+    q = np.concatenate([_f64(pr["charge"]) for pr in proteins])
+    rs = np.concatenate([_f64(pr["rstar"]) for pr in proteins])
+    ep = np.concatenate([_f64(pr["eps"]) for pr in proteins])
+    handles = [TopoHandle(c, int(n)) for c, n in zip(topo_csrs, n_atoms)]
+    topos = (Topo * n_prot)(*[h.struct for h in handles])
+    x = np.ascontiguousarray(np.concatenate([_f64(v).reshape(-1) for v in xyz]))
+    S = n_prot * n_rep
+    st = (PtStats * S)()
+    sw = np.zeros(2 * n_prot, np.int64)
+    T = _f64(temps)
+    tr = {}
+    if trace:
+        tr = {"bond": np.zeros(S * steps, np.int32), "delta": np.zeros(S * steps),
+              "dE": np.zeros(S * steps), "accepted": np.zeros(S * steps, np.int32),
+              "slot": np.zeros(S * steps, np.int32)}
+    rc = lib().orc_pt_run(n_prot, _p(n_atoms, C.c_int), _p(q, C.c_double), _p(rs, C.c_double),
+                          _p(ep, C.c_double), topos, n_rep, _p(T, C.c_double), _p(x, C.c_double),
+                          C.byref(PtParams(steps, exchange_every, mala_step, seed)), st,
+                          sw.ctypes.data_as(C.POINTER(C.c_int64)),
+                          _p(tr["bond"], C.c_int) if trace else None,
+                          _p(tr["delta"], C.c_double) if trace else None,
+                          _p(tr["dE"], C.c_double) if trace else None,
+                          _p(tr["accepted"], C.c_int) if trace else None,
+                          _p(tr["slot"], C.c_int) if trace else None)
+    if rc != 0:
+        raise ValueError("orc_pt_run rejected its arguments")
+    out_x, off = [], 0
+    for pr in range(n_prot):
+        for _ in range(n_rep):
+            m = 3 * int(n_atoms[pr])
+            out_x.append(x[off:off + m].reshape(-1, 3))
+            off += m
+    stats = [{f: getattr(v, f) for f, _ in PtStats._fields_ if f != "pad"} for v in st]
+    res = {"xyz": out_x, "stats": stats, "swaps": sw.reshape(-1, 2)}
+    if trace:
+        res["trace"] = {k: v.reshape(S, steps) for k, v in tr.items()}
+    return res
 
 
 def num_threads():

@@ -1,0 +1,124 @@
+"""The oracle's gradient-guided (MALA) torsion sampler and its replica
+exchange, checked where the mathematics fixes the answer:
+  * Philox4x32-10 against the published known-answer vectors (Random123);
+  * a one-torsion toy (four atoms, the last one rotating about the middle
+    bond): the sampled torsion distribution equals exp(-E(theta)/kT)
+    integrated by quadrature over theta, for MALA alone and for the slot-0
+    replica of a two-temperature exchange run;
+  * the per-step trace: each accepted step's dE equals the difference of
+    full energies of the states before and after.
+CPU only."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def test_philox_known_answers():
+    assert O.philox([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert O.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert O.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def _toy():
+    """A-B-C-D, rotatable bond B-C carrying D; A-D is the only non-excluded
+    pair; one dihedral record (A, B, C, D) with two Fourier terms."""
+    xyz = np.array([[-0.5, 1.4, 0.0], [0.0, 0.0, 0.0], [1.5, 0.0, 0.0], [2.0, 1.4, 0.0]])
+    s = {"xyz": xyz, "charge": np.array([0.5, 0.0, 0.0, -0.5]), "rstar": np.array([1.5, 1.6, 1.6, 1.5]),
+         "eps": np.array([0.1, 0.1, 0.1, 0.1])}
+    excl = [[1, 2], [2, 3], [3], []]
+    c = {"excl_off": np.array([0, 2, 4, 5, 5], np.int32), "excl_idx": np.array([j for r in excl for j in r], np.int32),
+         "rb_i": np.array([1], np.int32), "rb_j": np.array([2], np.int32), "rb_kind": np.array([2], np.int32),
+         "rb_scale": np.array([1.0]), "side_off": np.array([0, 1], np.int32), "side_idx": np.array([3], np.int32),
+         "dih_a": np.array([0], np.int32), "dih_b": np.array([1], np.int32), "dih_c": np.array([2], np.int32),
+         "dih_d": np.array([3], np.int32), "dih_toff": np.array([0, 2], np.int32),
+         "dih_v2": np.array([0.6, 0.3]), "dih_n": np.array([1, 3], np.int32), "dih_gamma": np.array([0.0, 0.5])}
+    return s, c
+
+
+def _rotate_d(xyz, theta):
+    o = xyz[1]
+    u = (xyz[2] - o) / np.linalg.norm(xyz[2] - o)
+    v = xyz[3] - o
+    out = xyz.copy()
+    out[3] = o + v * np.cos(theta) + np.cross(u, v) * np.sin(theta) + u * (u @ v) * (1 - np.cos(theta))
+    return out
+
+
+def _exact_density(s, c, kT, edges):
+    grid = np.linspace(-np.pi, np.pi, 4001)[:-1] + np.pi / 4000
+    e = np.array([O.energy_terms(dict(s, xyz=_rotate_d(s["xyz"], t)), c)["TOTAL"] for t in grid])
+    w = np.exp(-(e - e.min()) / kT)
+    hist = np.histogram(grid, bins=edges, weights=w)[0]
+    return hist / hist.sum()
+
+
+def _theta_path(delta, acc):
+    th = np.cumsum(np.where(acc == 1, delta, 0.0))
+    return np.angle(np.exp(1j * th))
+
+
+def test_mala_samples_the_boltzmann_distribution():
+    s, c = _toy()
+    kT = 0.8
+    steps = 60000
+    r = O.pt_run([s], [c], 1, [kT], [s["xyz"]], steps, mala_step=0.9, seed=11, trace=True)
+    tr = r["trace"]
+    acc = tr["accepted"][0]
+    assert 0.2 < acc.mean() < 0.98
+    th = _theta_path(tr["delta"][0], acc)[2000:]
+    edges = np.linspace(-np.pi, np.pi, 13)
+    got = np.histogram(th, bins=edges)[0] / len(th)
+    want = _exact_density(s, c, kT, edges)
+    assert np.max(np.abs(got - want)) < 0.02, (got, want)
+
+
+def test_accepted_dE_equals_energy_difference():
+    s, c = _toy()
+    r = O.pt_run([s], [c], 1, [0.6], [s["xyz"]], 40, mala_step=0.5, seed=3, trace=True)
+    tr = r["trace"]
+    x = s["xyz"].copy()
+    e_prev = O.energy_terms(dict(s, xyz=x), c)["TOTAL"]
+    for d, a, de in zip(tr["delta"][0], tr["accepted"][0], tr["dE"][0]):
+        xn = _rotate_d(x, d)
+        e_new = O.energy_terms(dict(s, xyz=xn), c)["TOTAL"]
+        assert abs((e_new - e_prev) - de) <= 1e-9 * max(1.0, abs(de))
+        if a:
+            x, e_prev = xn, e_new
+    assert np.allclose(x, r["xyz"][0], atol=1e-9)
+    assert abs(r["stats"][0]["e"] - e_prev) <= 1e-9 * abs(e_prev)
+
+
+def test_exchange_keeps_the_cold_slot_exact():
+    """Two replicas at kT 0.5 and 1.5 exchanging every 5 steps: the states
+    seen in slot 0 follow exp(-E/0.5)."""
+    s, c = _toy()
+    temps = [0.5, 1.5]
+    steps = 40000
+    r = O.pt_run([s], [c], 2, temps, [s["xyz"], s["xyz"]], steps, exchange_every=5, mala_step=0.8,
+                 seed=21, trace=True)
+    tr = r["trace"]
+    att, acc = r["swaps"][0]
+    assert att == (steps // 5 + 1) // 2   # pair (0, 1) is tried on even rounds only
+    assert 0 < acc < att
+    # the state during step t is the state after step t - 1
+    paths = [_theta_path(tr["delta"][k], tr["accepted"][k]) for k in range(2)]
+    cold = []
+    for t in range(2000, steps):
+        for k in range(2):
+            if tr["slot"][k][t] == 0:
+                cold.append(paths[k][t - 1])
+    edges = np.linspace(-np.pi, np.pi, 13)
+    got = np.histogram(cold, bins=edges)[0] / len(cold)
+    want = _exact_density(s, c, temps[0], edges)
+    assert np.max(np.abs(got - want)) < 0.025, (got, want)
+
+
+@pytest.mark.parametrize("bad", [dict(mala_step=0.0), dict(steps=-1)])
+def test_pt_rejects_bad_arguments(bad):
+    s, c = _toy()
+    kw = dict(steps=5, mala_step=0.1)
+    kw.update(bad)
+    with pytest.raises(ValueError):
+        O.pt_run([s], [c], 1, [0.6], [s["xyz"]], kw["steps"], mala_step=kw["mala_step"])
