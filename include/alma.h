@@ -135,6 +135,60 @@ alma_status alma_mc_timing(alma_ctx* ctx, double* out);
  * bond (torque about each bond for its j-side), at the loaded positions. */
 alma_status alma_forces(alma_ctx* ctx, double* energy, double* dEdx, double* dEdtheta);
 
+/* ---- batched replicas: parallel tempering with gradient-guided moves ----
+ * No reference counterpart as a device path; the exchange rule is the
+ * reference's parallel-tempering driver (python/gui_main.py:1960-2013:
+ * alternating neighbour pairs, swap with probability
+ * min(1, exp((E_a - E_b)(1/T_a - 1/T_b))), state and energy move together),
+ * the energy is calculate_potential (src/physics_engine.cpp:2242-2248).
+ * Each step of every system is one MALA torsion move: bond k uniform,
+ * delta = -(h^2/2) dE/dtheta_k / kT + h xi, Metropolis-Hastings with the
+ * reverse-move density, Philox4x32-10 draws keyed by {seed, system} and
+ * counted by step.  System s = p * n_rep + r starts in slot r. */
+typedef struct {
+    int32_t n_prot;            /* proteins                                   */
+    const int32_t* n_atoms;    /* atoms per protein                          */
+    const double* q;           /* concatenated per-protein parameters        */
+    const double* rstar;
+    const double* eps;
+    const alma_topo* topo;     /* n_prot topologies, local atom indices      */
+    int32_t n_rep;             /* temperature slots (replicas) per protein   */
+    const double* temps;       /* n_rep kT values (kcal/mol)                 */
+} alma_pt_systems;
+
+typedef struct {
+    int32_t steps;             /* MALA steps per system in this call         */
+    int32_t exchange_every;    /* steps between exchange rounds (0: none)    */
+    double mala_step;          /* h, rad                                     */
+    uint32_t seed;
+    int32_t trace;             /* keep a per-step trace for alma_pt_trace    */
+} alma_pt_params;
+
+typedef struct {
+    double e;                  /* energy of the current state                */
+    int64_t accepted;
+    int64_t moves;
+    int32_t slot;              /* temperature slot                           */
+    int32_t status;            /* 0 ok                                       */
+} alma_pt_stats;
+
+/* Load the systems; xyz holds all systems (system-major, n_p x 3 each) and
+ * the initial states are evaluated (energy and gradient). */
+alma_status alma_pt_set(alma_ctx* ctx, const alma_pt_systems* sys, const double* xyz);
+/* Advance every system by p->steps MALA steps (continuing from the current
+ * states and step counters); xyz_out (may be NULL) receives the states,
+ * st one entry per system, swaps 2 per protein (attempted, accepted). */
+alma_status alma_pt_run(alma_ctx* ctx, const alma_pt_params* p, double* xyz_out, alma_pt_stats* st,
+                        int64_t* swaps);
+/* Per-step trace of the last alma_pt_run with trace set: steps per system,
+ * system-major. */
+alma_status alma_pt_trace(alma_ctx* ctx, int32_t* bond, double* delta, double* dE, int32_t* accepted);
+/* Per-term energies (8 per system) of the current states, fresh evaluation. */
+alma_status alma_pt_energy(alma_ctx* ctx, double* terms);
+/* out[4] = {device ms of the last alma_pt_run, steps, atoms, lanes per row
+ * of the pair kernel chosen by the autotuner}. */
+alma_status alma_pt_timing(alma_ctx* ctx, double* out);
+
 #ifdef __cplusplus
 }
 #endif

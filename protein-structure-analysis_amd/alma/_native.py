@@ -46,6 +46,21 @@ class McStats(C.Structure):
                 ("moves", C.c_int64), ("rebuilds", C.c_int64), ("words", C.c_int64)]
 
 
+class PtSystems(C.Structure):
+    _fields_ = [("n_prot", C.c_int32), ("n_atoms", _I), ("q", _D), ("rstar", _D), ("eps", _D),
+                ("topo", C.POINTER(Topo)), ("n_rep", C.c_int32), ("temps", _D)]
+
+
+class PtParams(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("exchange_every", C.c_int32), ("mala_step", C.c_double),
+                ("seed", C.c_uint32), ("trace", C.c_int32)]
+
+
+class PtStats(C.Structure):
+    _fields_ = [("e", C.c_double), ("accepted", C.c_int64), ("moves", C.c_int64),
+                ("slot", C.c_int32), ("status", C.c_int32)]
+
+
 _LIB = None
 _WITH_TORCH_RUNTIME = False
 
@@ -81,13 +96,19 @@ def lib():
         L.alma_mc_trace.argtypes = [P, _I, _I, _D, _D, _I, _D]
         L.alma_forces.argtypes = [P, _D, _D, _D]
         L.alma_mc_timing.argtypes = [P, _D]
+        L.alma_pt_set.argtypes = [P, C.POINTER(PtSystems), _D]
+        L.alma_pt_run.argtypes = [P, C.POINTER(PtParams), _D, C.POINTER(PtStats), C.POINTER(C.c_int64)]
+        L.alma_pt_trace.argtypes = [P, _I, _D, _D, _I]
+        L.alma_pt_energy.argtypes = [P, _D]
+        L.alma_pt_timing.argtypes = [P, _D]
         _LIB = L
     return _LIB
 
 
 EXPORTED = ("alma_create", "alma_destroy", "alma_last_error", "alma_device_name",
             "alma_set_system", "alma_set_positions", "alma_energy", "alma_nlist",
-            "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_mc_timing", "alma_forces")
+            "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_mc_timing", "alma_forces",
+            "alma_pt_set", "alma_pt_run", "alma_pt_trace", "alma_pt_energy", "alma_pt_timing")
 
 
 def _p(a, t):
@@ -238,3 +259,58 @@ class Context:
         self._check(lib().alma_forces(self._h, C.byref(e), _p(g, C.c_double),
                                       _p(gt, C.c_double) if nrot else None))
         return e.value, g, gt[:nrot]
+
+
+    # ---- batched replicas: parallel tempering with MALA torsion moves ----
+    def pt_set(self, proteins, topo_csrs, n_rep, temps, xyz):
+        """proteins: system dicts (charge / rstar / eps used); topo_csrs: one CSR
+        topology per protein; xyz: n_prot * n_rep coordinate arrays, system
+        s = p * n_rep + r."""
+        self._pt_n = [len(pr["charge"]) for pr in proteins]
+        self._pt_rep = n_rep
+        n_atoms = np.array(self._pt_n, np.int32)
+        arrs = [n_atoms] + [np.ascontiguousarray(np.concatenate([np.asarray(pr[k], np.float64) for pr in proteins]))
+                            for k in ("charge", "rstar", "eps")]
+        topos = [TopoArrays(c, n) for c, n in zip(topo_csrs, self._pt_n)]
+        tarr = (Topo * len(topos))(*[t.struct for t in topos])
+        T = np.ascontiguousarray(temps, np.float64)
+        x = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64).reshape(-1) for v in xyz]))
+        ps = PtSystems(len(proteins), _p(arrs[0], C.c_int32), _p(arrs[1], C.c_double), _p(arrs[2], C.c_double),
+                       _p(arrs[3], C.c_double), tarr, n_rep, _p(T, C.c_double))
+        self._check(lib().alma_pt_set(self._h, C.byref(ps), _p(x, C.c_double)))
+        self._pt_keep = (arrs, topos, tarr, T)
+
+    def pt_run(self, steps, exchange_every=0, mala_step=0.02, seed=1, trace=False):
+        S = len(self._pt_n) * self._pt_rep
+        total = sum(n * self._pt_rep for n in self._pt_n)
+        x = np.zeros(3 * total)
+        st = (PtStats * S)()
+        sw = np.zeros(2 * len(self._pt_n), np.int64)
+        self._check(lib().alma_pt_run(self._h, C.byref(PtParams(steps, exchange_every, mala_step, seed,
+                                                                   1 if trace else 0)),
+                                      _p(x, C.c_double), st, sw.ctypes.data_as(C.POINTER(C.c_int64))))
+        out_x, off = [], 0
+        for n in self._pt_n:
+            for _ in range(self._pt_rep):
+                out_x.append(x[off:off + 3 * n].reshape(-1, 3))
+                off += 3 * n
+        stats = [{f: getattr(v, f) for f, _ in PtStats._fields_} for v in st]
+        res = {"xyz": out_x, "stats": stats, "swaps": sw.reshape(-1, 2)}
+        if trace:
+            tr = {"bond": np.zeros(S * steps, np.int32), "delta": np.zeros(S * steps),
+                  "dE": np.zeros(S * steps), "accepted": np.zeros(S * steps, np.int32)}
+            self._check(lib().alma_pt_trace(self._h, _p(tr["bond"], C.c_int32), _p(tr["delta"], C.c_double),
+                                            _p(tr["dE"], C.c_double), _p(tr["accepted"], C.c_int32)))
+            res["trace"] = {k: v.reshape(S, steps) for k, v in tr.items()}
+        return res
+
+    def pt_energy(self):
+        S = len(self._pt_n) * self._pt_rep
+        out = np.zeros(8 * S)
+        self._check(lib().alma_pt_energy(self._h, _p(out, C.c_double)))
+        return out.reshape(S, 8)
+
+    def pt_timing(self) -> dict:
+        out = np.zeros(4)
+        self._check(lib().alma_pt_timing(self._h, _p(out, C.c_double)))
+        return dict(zip(("device_ms", "steps", "atoms", "pair_lanes"), out.tolist()))

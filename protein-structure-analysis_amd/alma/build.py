@@ -22,7 +22,7 @@ EXT = sysconfig.get_config_var("EXT_SUFFIX")
 MODULE = os.path.join(PKG, "protein_physics" + EXT)
 MODULE_GPU_NAME = os.path.join(PKG, "protein_physics_cuda" + EXT)
 
-HIP_SOURCES = ["energy.hip", "mc.hip", "forces.hip", "alma_abi.cpp"]
+HIP_SOURCES = ["energy.hip", "mc.hip", "forces.hip", "batch.hip", "alma_abi.cpp"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                "-munsafe-fp-atomics", "-Wno-unused-result"]

@@ -608,7 +608,506 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     ctx->have_eval = false;
 }
 
+// ───────────────────── batched replicas (PT / MALA) ─────────────────────────
+void free_batch(alma_ctx* ctx) {
+    Batch& b = ctx->batch;
+    dev_free(b.prot);
+    dev_free(b.sys);
+    dev_free(b.st);
+    dev_free(b.grid);
+    dev_free(b.atom_sys);
+    dev_free(b.pos);
+    dev_free(b.posn);
+    dev_free(b.prm);
+    dev_free(b.rstar);
+    dev_free(b.eps64);
+    dev_free(b.a);
+    dev_free(b.born_sum);
+    dev_free(b.dEda);
+    dev_free(b.G);
+    dev_free(b.Gn);
+    dev_free(b.part);
+    dev_free(b.cell_of);
+    dev_free(b.cell_tmp);
+    dev_free(b.cell_atoms);
+    dev_free(b.cell_cnt);
+    dev_free(b.cell_start);
+    dev_free(b.nl);
+    dev_free(b.nl_cnt);
+    dev_free(b.flags);
+    dev_free(b.excl_off);
+    dev_free(b.excl_idx);
+    dev_free(b.rb_i);
+    dev_free(b.rb_j);
+    dev_free(b.run_off);
+    dev_free(b.side_run);
+    dev_free(b.dih_a);
+    dev_free(b.dih_b);
+    dev_free(b.dih_c);
+    dev_free(b.dih_d);
+    dev_free(b.dih_toff);
+    dev_free(b.dih_v2);
+    dev_free(b.dih_n);
+    dev_free(b.dih_gamma);
+    dev_free(b.ss_i);
+    dev_free(b.ss_j);
+    dev_free(b.rec_sys);
+    dev_free(b.rec_e);
+    dev_free(b.terms);
+    dev_free(b.slot_sys);
+    dev_free(b.temps);
+    dev_free(b.swaps);
+    dev_free(b.tr_bond);
+    dev_free(b.tr_delta);
+    dev_free(b.tr_dE);
+    dev_free(b.tr_acc);
+    b = Batch{};
+    ctx->h_bsys.clear();
+    ctx->h_bprot.clear();
+    ctx->pt_ready = false;
+    ctx->pt_step = 0;
+}
+
+template <class T>
+void up(T** p, const std::vector<T>& v) {
+    dev_upload(p, v.data(), v.size());
+}
+
+// initial evaluation (energy and gradient of the loaded states); returns the
+// longest neighbour row seen
+int batch_init_eval(alma_ctx* ctx) {
+    Batch& b = ctx->batch;
+    HIP_TRY(hipMemsetAsync(b.flags, 0, sizeof(int), ctx->stream));
+    HIP_TRY(launch_batch_step(b, 1, ctx->stream));
+    int maxrow = 0;
+    HIP_TRY(hipMemcpyAsync(&maxrow, b.flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return maxrow;
+}
+
+void batch_states(alma_ctx* ctx, std::vector<BState>& st) {
+    st.resize(ctx->batch.n_sys);
+    HIP_TRY(hipMemcpy(st.data(), ctx->batch.st, sizeof(BState) * st.size(), hipMemcpyDeviceToHost));
+}
+
+void batch_check(alma_ctx* ctx) {
+    std::vector<BState> st;
+    batch_states(ctx, st);
+    for (const BState& v : st) {
+        if (v.status == 1) throw Fail{ALMA_ERUNTIME, "neighbour rows exceeded their capacity"};
+        if (v.status == 2) throw Fail{ALMA_ERUNTIME, "cell grid exceeds its allocation"};
+    }
+}
+
+// pair-kernel variant by timing: lanes per atom row x block size
+void batch_autotune(alma_ctx* ctx) {
+    Batch& b = ctx->batch;
+    if (!ctx->ev0) {
+        HIP_TRY(hipEventCreate(&ctx->ev0));
+        HIP_TRY(hipEventCreate(&ctx->ev1));
+    }
+    const int lanes[3] = {16, 32, 64};
+    const int blocks[2] = {256, 512};
+    float best = 1e30f;
+    for (int L : lanes)
+        for (int B : blocks) {
+            HIP_TRY(launch_batch_pair(b, L, B, ctx->stream));   // warm
+            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+            for (int r = 0; r < 3; ++r) HIP_TRY(launch_batch_pair(b, L, B, ctx->stream));
+            HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+            HIP_TRY(hipEventSynchronize(ctx->ev1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ms < best) {
+                best = ms;
+                b.lanes = L;
+                b.pair_block = B;
+            }
+        }
+}
+
+void pt_set(alma_ctx* ctx, const alma_pt_systems* ps, const double* xyz) {
+    if (!ps || !xyz) throw Fail{ALMA_EINVAL, "systems or coordinates missing"};
+    const int P = ps->n_prot, R = ps->n_rep;
+    if (P <= 0 || R <= 0) throw Fail{ALMA_EINVAL, "n_prot and n_rep must be positive"};
+    if (!ps->n_atoms || !ps->q || !ps->rstar || !ps->eps || !ps->topo || !ps->temps)
+        throw Fail{ALMA_EINVAL, "system arrays missing"};
+    for (int r = 0; r < R; ++r)
+        if (!(ps->temps[r] > 0.0)) throw Fail{ALMA_EINVAL, "temperature must be positive"};
+    long long total = 0;
+    size_t poff = 0;
+    for (int p = 0; p < P; ++p) {
+        int n = ps->n_atoms[p];
+        if (n <= 0) throw Fail{ALMA_EINVAL, "every protein needs atoms"};
+        validate_topo(&ps->topo[p], n);
+        if (ps->topo[p].n_rot <= 0) throw Fail{ALMA_EINVAL, "topology has no rotatable bonds"};
+        for (int i = 0; i < n; ++i)
+            if (!(ps->rstar[poff + i] > 0.0)) throw Fail{ALMA_EINVAL, "particle radius must be positive"};
+        poff += n;
+        total += (long long)n * R;
+    }
+    if (total > (1LL << 30)) throw Fail{ALMA_EINVAL, "too many atoms for one batch"};
+    HIP_TRY(hipSetDevice(ctx->device));
+    free_batch(ctx);
+    Batch& b = ctx->batch;
+    b.n_prot = P;
+    b.n_rep = R;
+    b.n_sys = P * R;
+    b.n_atoms = (int)total;
+    // topology, packed with local indices
+    std::vector<BProt> prot(P);
+    std::vector<int> excl_off, excl_idx, rb_i, rb_j, run_off, dih_a, dih_b, dih_c, dih_d, dih_toff, dih_n, ss_i, ss_j;
+    std::vector<int2> runs;
+    std::vector<double> dih_v2, dih_gamma;
+    for (int p = 0; p < P; ++p) {
+        const alma_topo* t = &ps->topo[p];
+        const int n = ps->n_atoms[p];
+        BProt& pr = prot[p];
+        pr.n = n;
+        pr.excl_base = (int)excl_off.size();
+        const int eb = (int)excl_idx.size();
+        for (int i = 0; i <= n; ++i) excl_off.push_back(eb + t->excl_off[i]);
+        excl_idx.insert(excl_idx.end(), t->excl_idx, t->excl_idx + t->excl_off[n]);
+        pr.rot_off = (int)rb_i.size();
+        pr.n_rot = t->n_rot;
+        rb_i.insert(rb_i.end(), t->rb_i, t->rb_i + t->n_rot);
+        rb_j.insert(rb_j.end(), t->rb_j, t->rb_j + t->n_rot);
+        std::vector<int2> r;
+        std::vector<int> ro;
+        side_runs(t, r, ro);
+        const int rb = (int)runs.size();
+        for (int k = 0; k < t->n_rot; ++k) run_off.push_back(rb + ro[k]);
+        runs.insert(runs.end(), r.begin(), r.end());
+        pr.dih_off = (int)dih_a.size();
+        pr.n_dih = t->n_dih;
+        const int tb = (int)dih_v2.size();
+        for (int k = 0; k < t->n_dih; ++k) {
+            dih_a.push_back(t->dih_a[k]);
+            dih_b.push_back(t->dih_b[k]);
+            dih_c.push_back(t->dih_c[k]);
+            dih_d.push_back(t->dih_d[k]);
+            dih_toff.push_back(tb + t->dih_toff[k]);
+        }
+        if (t->n_dih) {
+            int nt = t->dih_toff[t->n_dih];
+            dih_v2.insert(dih_v2.end(), t->dih_v2, t->dih_v2 + nt);
+            dih_n.insert(dih_n.end(), t->dih_n, t->dih_n + nt);
+            dih_gamma.insert(dih_gamma.end(), t->dih_gamma, t->dih_gamma + nt);
+        }
+        pr.ss_off = (int)ss_i.size();
+        pr.n_ss = t->n_ss;
+        ss_i.insert(ss_i.end(), t->ss_i, t->ss_i + t->n_ss);
+        ss_j.insert(ss_j.end(), t->ss_j, t->ss_j + t->n_ss);
+    }
+    run_off.push_back((int)runs.size());
+    dih_toff.push_back((int)dih_v2.size());
+    // systems, atoms, records
+    std::vector<BSys> sys(b.n_sys);
+    std::vector<BState> st(b.n_sys);
+    std::vector<int> atom_sys(total), rec_sys, slot_sys(b.n_sys);
+    std::vector<double> pos(4 * (size_t)total, 0.0), rs(total), ep(total);
+    std::vector<float4> prm(total);
+    long long aoff = 0, coff = 0;
+    size_t xoff = 0;
+    poff = 0;
+    for (int p = 0; p < P; ++p) {
+        const int n = ps->n_atoms[p];
+        for (int r = 0; r < R; ++r) {
+            const int s = p * R + r;
+            BSys& sy = sys[s];
+            sy.prot = p;
+            sy.n = n;
+            sy.aoff = (int)aoff;
+            sy.coff = (int)coff;
+            std::vector<double> x(xyz + xoff, xyz + xoff + 3 * (size_t)n);
+            sy.max_cells = cells_needed(x, n);
+            coff += sy.max_cells + 1;
+            sy.roff = (int)rec_sys.size();
+            for (int k = 0; k < prot[p].n_dih + prot[p].n_ss; ++k) rec_sys.push_back(s);
+            for (int i = 0; i < n; ++i) {
+                for (int d = 0; d < 3; ++d) pos[4 * (aoff + i) + d] = x[3 * i + d];
+                prm[aoff + i] = make_float4((float)ps->q[poff + i], (float)ps->rstar[poff + i],
+                                            (float)ps->eps[poff + i], 0.f);
+                rs[aoff + i] = ps->rstar[poff + i];
+                ep[aoff + i] = ps->eps[poff + i];
+                atom_sys[aoff + i] = s;
+            }
+            BState& v = st[s];
+            std::memset(&v, 0, sizeof(v));
+            v.temperature = ps->temps[r];
+            v.slot = r;
+            slot_sys[s] = s;
+            aoff += n;
+            xoff += 3 * (size_t)n;
+        }
+        poff += n;
+    }
+    if (coff > (1LL << 30)) throw Fail{ALMA_EINVAL, "cell grids too large for one batch"};
+    b.n_rec = (int)rec_sys.size();
+    b.cells = (int)coff;
+    up(&b.prot, prot);
+    up(&b.sys, sys);
+    up(&b.st, st);
+    dev_alloc(&b.grid, b.n_sys);
+    up(&b.atom_sys, atom_sys);
+    up(&b.pos, pos);
+    dev_alloc(&b.posn, 4 * (size_t)total);
+    up(&b.prm, prm);
+    up(&b.rstar, rs);
+    up(&b.eps64, ep);
+    dev_alloc(&b.a, total);
+    dev_alloc(&b.born_sum, total);
+    dev_alloc(&b.dEda, total);
+    dev_alloc(&b.G, 3 * (size_t)total);
+    dev_alloc(&b.Gn, 3 * (size_t)total);
+    dev_alloc(&b.part, 5 * (size_t)total);
+    dev_alloc(&b.cell_of, total);
+    dev_alloc(&b.cell_tmp, total);
+    dev_alloc(&b.cell_atoms, total);
+    dev_alloc(&b.cell_cnt, coff);
+    dev_alloc(&b.cell_start, coff);
+    dev_alloc(&b.nl_cnt, total);
+    dev_alloc(&b.flags, 4);
+    up(&b.excl_off, excl_off);
+    up(&b.excl_idx, excl_idx);
+    up(&b.rb_i, rb_i);
+    up(&b.rb_j, rb_j);
+    up(&b.run_off, run_off);
+    up(&b.side_run, runs);
+    up(&b.dih_a, dih_a);
+    up(&b.dih_b, dih_b);
+    up(&b.dih_c, dih_c);
+    up(&b.dih_d, dih_d);
+    up(&b.dih_toff, dih_toff);
+    up(&b.dih_v2, dih_v2);
+    up(&b.dih_n, dih_n);
+    up(&b.dih_gamma, dih_gamma);
+    up(&b.ss_i, ss_i);
+    up(&b.ss_j, ss_j);
+    up(&b.rec_sys, rec_sys);
+    dev_alloc(&b.rec_e, std::max(b.n_rec, 1));
+    dev_alloc(&b.terms, 8 * (size_t)b.n_sys);
+    up(&b.slot_sys, slot_sys);
+    std::vector<double> temps(ps->temps, ps->temps + R);
+    up(&b.temps, temps);
+    std::vector<long long> sw(2 * (size_t)P, 0);
+    up(&b.swaps, sw);
+    ctx->h_bsys = sys;
+    ctx->h_bprot = prot;
+    // size the rows from a first evaluation, then evaluate for real
+    b.cap = 256;
+    dev_alloc(&b.nl, (size_t)total * b.cap);
+    int maxrow = batch_init_eval(ctx);
+    b.cap = (maxrow * 3 / 2 + 64 + 63) / 64 * 64;
+    dev_alloc(&b.nl, (size_t)total * b.cap);
+    up(&b.st, st);   // clear the overflow status of the sizing pass
+    batch_init_eval(ctx);
+    batch_autotune(ctx);
+    batch_init_eval(ctx);
+    batch_check(ctx);
+    ctx->pt_step = 0;
+    ctx->pt_ready = true;
+}
+
+void pt_run(alma_ctx* ctx, const alma_pt_params* p, double* xyz_out, alma_pt_stats* out, int64_t* swaps) {
+    if (!ctx->pt_ready) throw Fail{ALMA_EINVAL, "no batched systems loaded"};
+    if (!p) throw Fail{ALMA_EINVAL, "parameters missing"};
+    if (p->steps < 0 || p->exchange_every < 0) throw Fail{ALMA_EINVAL, "steps and exchange interval must be >= 0"};
+    if (!(p->mala_step > 0.0) || !std::isfinite(p->mala_step)) throw Fail{ALMA_EINVAL, "MALA step must be positive"};
+    HIP_TRY(hipSetDevice(ctx->device));
+    Batch& b = ctx->batch;
+    b.h = p->mala_step;
+    b.seed = p->seed;
+    b.exchange_every = p->exchange_every;
+    dev_free(b.tr_bond);
+    dev_free(b.tr_delta);
+    dev_free(b.tr_dE);
+    dev_free(b.tr_acc);
+    b.trace_cap = p->trace ? p->steps : 0;
+    b.trace_base = ctx->pt_step;
+    if (b.trace_cap) {
+        size_t m = (size_t)b.n_sys * b.trace_cap;
+        dev_alloc(&b.tr_bond, m);
+        dev_alloc(&b.tr_delta, m);
+        dev_alloc(&b.tr_dE, m);
+        dev_alloc(&b.tr_acc, m);
+    }
+    ctx->t_pt_ms = 0.0;
+    ctx->pt_replays = 0;
+    if (p->steps > 0) {
+        // one step as a graph (fixed kernel sequence, arguments by value)
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = launch_batch_step(b, 0, ctx->stream);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        HIP_TRY(le);
+        HIP_TRY(ce);
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            hipGraphDestroy(graph);
+            HIP_TRY(ie);
+        }
+        if (!ctx->ev0) {
+            HIP_TRY(hipEventCreate(&ctx->ev0));
+            HIP_TRY(hipEventCreate(&ctx->ev1));
+        }
+        hipError_t re = hipEventRecord(ctx->ev0, ctx->stream);
+        for (int k = 0; k < p->steps && re == hipSuccess; ++k) re = hipGraphLaunch(exec, ctx->stream);
+        if (re == hipSuccess) re = hipEventRecord(ctx->ev1, ctx->stream);
+        if (re == hipSuccess) re = hipEventSynchronize(ctx->ev1);
+        float ms = 0.f;
+        if (re == hipSuccess) re = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        HIP_TRY(re);
+        ctx->t_pt_ms = ms;
+        ctx->pt_replays = p->steps;
+    }
+    std::vector<BState> st;
+    batch_states(ctx, st);
+    ctx->pt_step = st.empty() ? 0 : st[0].step;
+    for (const BState& v : st)
+        if (v.status != 0) ctx->pt_step = std::max(ctx->pt_step, v.step);
+    if (out)
+        for (int s = 0; s < b.n_sys; ++s) {
+            out[s].e = st[s].e;
+            out[s].accepted = st[s].accepted;
+            out[s].moves = st[s].moves;
+            out[s].slot = st[s].slot;
+            out[s].status = st[s].status;
+        }
+    if (swaps) {
+        std::vector<long long> sw(2 * (size_t)b.n_prot);
+        HIP_TRY(hipMemcpy(sw.data(), b.swaps, sizeof(long long) * sw.size(), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < sw.size(); ++k) swaps[k] = sw[k];
+    }
+    if (xyz_out) {
+        std::vector<double> p4(4 * (size_t)b.n_atoms);
+        HIP_TRY(hipMemcpy(p4.data(), b.pos, sizeof(double) * p4.size(), hipMemcpyDeviceToHost));
+        for (long long i = 0; i < b.n_atoms; ++i)
+            for (int d = 0; d < 3; ++d) xyz_out[3 * i + d] = p4[4 * i + d];
+    }
+    ctx->pt_tr_steps = b.trace_cap;
+    if (b.trace_cap) {
+        size_t m = (size_t)b.n_sys * b.trace_cap;
+        ctx->pt_tr_bond.resize(m);
+        ctx->pt_tr_acc.resize(m);
+        ctx->pt_tr_delta.resize(m);
+        ctx->pt_tr_dE.resize(m);
+        HIP_TRY(hipMemcpy(ctx->pt_tr_bond.data(), b.tr_bond, sizeof(int) * m, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ctx->pt_tr_acc.data(), b.tr_acc, sizeof(int) * m, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ctx->pt_tr_delta.data(), b.tr_delta, sizeof(double) * m, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ctx->pt_tr_dE.data(), b.tr_dE, sizeof(double) * m, hipMemcpyDeviceToHost));
+    }
+    batch_check(ctx);
+}
+
+// MALA through alma_mc_run: one protein (the loaded system), n_chains
+// replicas at one temperature, no exchanges; chain c is system c
+void run_mala(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_stats* stats) {
+    DevSys& s = ctx->sys;
+    if (s.n == 0) throw Fail{ALMA_EINVAL, "initial_state empty"};
+    if (!ctx->has_topo_copy || ctx->topo_copy.n_rot <= 0) throw Fail{ALMA_EINVAL, "topology has no rotatable bonds"};
+    if (p.n_chains <= 0 || p.steps <= 0) throw Fail{ALMA_EINVAL, "ncand/steps must be positive"};
+    if (!(p.temperature > 0.0)) throw Fail{ALMA_EINVAL, "temperature must be positive"};
+    const int n = s.n, nch = p.n_chains;
+    alma_topo tv = ctx->topo_copy.view();
+    std::vector<double> temps(nch, p.temperature), xyz;
+    xyz.reserve(3 * (size_t)n * nch);
+    for (int c = 0; c < nch; ++c) xyz.insert(xyz.end(), ctx->h_pos.begin(), ctx->h_pos.end());
+    int32_t na = n;
+    alma_pt_systems ps{1, &na, ctx->h_q.data(), ctx->h_r.data(), ctx->h_e.data(), &tv, nch, temps.data()};
+    pt_set(ctx, &ps, xyz.data());
+    alma_pt_params pp{p.steps, 0, p.mala_step, p.seed, p.trace};
+    std::vector<alma_pt_stats> st(nch);
+    pt_run(ctx, &pp, xyz_out, st.data(), nullptr);
+    for (int c = 0; c < nch; ++c) {
+        stats[c].cur_e = st[c].e;
+        stats[c].cur_max = p.mala_step;
+        stats[c].accepted = st[c].accepted;
+        stats[c].moves = st[c].moves;
+        stats[c].rebuilds = 0;
+        stats[c].words = 0;
+    }
+    if (p.trace) {
+        // chain 0's steps in the MC trace
+        ctx->tr_kind.assign(p.steps, 3);
+        ctx->tr_bond.assign(ctx->pt_tr_bond.begin(), ctx->pt_tr_bond.begin() + p.steps);
+        ctx->tr_delta.assign(ctx->pt_tr_delta.begin(), ctx->pt_tr_delta.begin() + p.steps);
+        ctx->tr_dE.assign(ctx->pt_tr_dE.begin(), ctx->pt_tr_dE.begin() + p.steps);
+        ctx->tr_acc.assign(ctx->pt_tr_acc.begin(), ctx->pt_tr_acc.begin() + p.steps);
+        ctx->tr_parts.assign(4 * (size_t)p.steps, 0.0);
+    }
+}
+
 }  // namespace
+
+void HostTopo::copy_from(const alma_topo* t) {
+    n_atoms = t->n_atoms;
+    n_rot = t->n_rot;
+    n_dih = t->n_dih;
+    n_ss = t->n_ss;
+    n_cp = t->n_cp;
+    n_csp = t->n_csp;
+    auto cp = [](std::vector<int32_t>& v, const int32_t* p, long m) { v.assign(p, p + (p ? m : 0)); };
+    cp(excl_off, t->excl_off, n_atoms + 1);
+    cp(excl_idx, t->excl_idx, t->excl_off[n_atoms]);
+    cp(rb_i, t->rb_i, n_rot);
+    cp(rb_j, t->rb_j, n_rot);
+    cp(rb_kind, t->rb_kind, n_rot);
+    rb_scale.assign(t->rb_scale, t->rb_scale + (t->rb_scale ? n_rot : 0));
+    cp(side_off, t->side_off, n_rot + 1);
+    cp(side_idx, t->side_idx, n_rot ? t->side_off[n_rot] : 0);
+    cp(dih_a, t->dih_a, n_dih);
+    cp(dih_b, t->dih_b, n_dih);
+    cp(dih_c, t->dih_c, n_dih);
+    cp(dih_d, t->dih_d, n_dih);
+    cp(dih_toff, t->dih_toff, n_dih ? n_dih + 1 : 0);
+    long nt = n_dih ? t->dih_toff[n_dih] : 0;
+    dih_v2.assign(t->dih_v2, t->dih_v2 + (t->dih_v2 ? nt : 0));
+    cp(dih_n, t->dih_n, nt);
+    dih_gamma.assign(t->dih_gamma, t->dih_gamma + (t->dih_gamma ? nt : 0));
+    cp(ss_i, t->ss_i, n_ss);
+    cp(ss_j, t->ss_j, n_ss);
+    cp(cp_phi, t->cp_phi, n_cp);
+    cp(cp_psi, t->cp_psi, n_cp);
+    cp(csp_a, t->csp_a, n_csp);
+    cp(csp_b, t->csp_b, n_csp);
+}
+
+alma_topo HostTopo::view() const {
+    alma_topo t{};
+    t.n_atoms = n_atoms;
+    t.excl_off = excl_off.data();
+    t.excl_idx = excl_idx.data();
+    t.n_rot = n_rot;
+    t.rb_i = rb_i.data();
+    t.rb_j = rb_j.data();
+    t.rb_kind = rb_kind.data();
+    t.rb_scale = rb_scale.data();
+    t.side_off = side_off.data();
+    t.side_idx = side_idx.data();
+    t.n_dih = n_dih;
+    t.dih_a = dih_a.data();
+    t.dih_b = dih_b.data();
+    t.dih_c = dih_c.data();
+    t.dih_d = dih_d.data();
+    t.dih_toff = dih_toff.data();
+    t.dih_v2 = dih_v2.data();
+    t.dih_n = dih_n.data();
+    t.dih_gamma = dih_gamma.data();
+    t.n_ss = n_ss;
+    t.ss_i = ss_i.data();
+    t.ss_j = ss_j.data();
+    t.n_cp = n_cp;
+    t.cp_phi = cp_phi.data();
+    t.cp_psi = cp_psi.data();
+    t.n_csp = n_csp;
+    t.csp_a = csp_a.data();
+    t.csp_b = csp_b.data();
+    return t;
+}
 
 extern "C" {
 
@@ -641,6 +1140,7 @@ void alma_destroy(alma_ctx* ctx) {
     free_sys(ctx->sys);
     free_chains(ctx->chains);
     free_forces(ctx);
+    free_batch(ctx);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -701,6 +1201,8 @@ alma_status alma_set_system(alma_ctx* ctx, int32_t n, const double* xyz, const d
         Grid g{};
         g.cs = std::sqrt(kNlRcut2) + 1e-6;
         dev_upload(&s.grid, &g, 1);
+        ctx->has_topo_copy = topo != nullptr;
+        if (topo) ctx->topo_copy.copy_from(topo);
         if (topo) {
             upload_topo(s.topo, topo, n);
             s.has_topo = true;
@@ -772,6 +1274,11 @@ alma_status alma_mc_run(alma_ctx* ctx, const alma_mc_params* p, double* xyz_out,
     return guarded(ctx, [&] {
         if (!p || !xyz_out || !stats) throw Fail{ALMA_EINVAL, "MC arguments missing"};
         HIP_TRY(hipSetDevice(ctx->device));
+        if (p->mode == ALMA_MC_MALA) {
+            if (p->snapshot_every) throw Fail{ALMA_EINVAL, "MALA runs take no snapshots"};
+            run_mala(ctx, *p, xyz_out, stats);
+            return;
+        }
         run_mc(ctx, *p, xyz_out, stats, snap_xyz, snap_e);
     });
 }
@@ -827,6 +1334,45 @@ alma_status alma_forces(alma_ctx* ctx, double* energy, double* dEdx, double* dEd
             HIP_TRY(hipMemcpyAsync(dEdtheta, ctx->f_dEdt, sizeof(double) * s.topo.n_rot, hipMemcpyDeviceToHost,
                                    ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+alma_status alma_pt_set(alma_ctx* ctx, const alma_pt_systems* sys, const double* xyz) {
+    return guarded(ctx, [&] { pt_set(ctx, sys, xyz); });
+}
+
+alma_status alma_pt_run(alma_ctx* ctx, const alma_pt_params* p, double* xyz_out, alma_pt_stats* st,
+                        int64_t* swaps) {
+    return guarded(ctx, [&] { pt_run(ctx, p, xyz_out, st, swaps); });
+}
+
+alma_status alma_pt_trace(alma_ctx* ctx, int32_t* bond, double* delta, double* dE, int32_t* accepted) {
+    return guarded(ctx, [&] {
+        if (bond) std::copy(ctx->pt_tr_bond.begin(), ctx->pt_tr_bond.end(), bond);
+        if (delta) std::copy(ctx->pt_tr_delta.begin(), ctx->pt_tr_delta.end(), delta);
+        if (dE) std::copy(ctx->pt_tr_dE.begin(), ctx->pt_tr_dE.end(), dE);
+        if (accepted) std::copy(ctx->pt_tr_acc.begin(), ctx->pt_tr_acc.end(), accepted);
+    });
+}
+
+alma_status alma_pt_energy(alma_ctx* ctx, double* terms) {
+    return guarded(ctx, [&] {
+        if (!terms) throw Fail{ALMA_EINVAL, "terms buffer missing"};
+        if (!ctx->pt_ready) throw Fail{ALMA_EINVAL, "no batched systems loaded"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        batch_init_eval(ctx);
+        batch_check(ctx);
+        HIP_TRY(hipMemcpy(terms, ctx->batch.terms, sizeof(double) * 8 * ctx->batch.n_sys, hipMemcpyDeviceToHost));
+    });
+}
+
+alma_status alma_pt_timing(alma_ctx* ctx, double* out) {
+    return guarded(ctx, [&] {
+        if (!out) throw Fail{ALMA_EINVAL, "output buffer missing"};
+        out[0] = ctx->t_pt_ms;
+        out[1] = (double)ctx->pt_replays;
+        out[2] = (double)ctx->batch.n_atoms;
+        out[3] = (double)ctx->batch.lanes;
     });
 }
 

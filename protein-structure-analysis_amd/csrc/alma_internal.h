@@ -165,6 +165,106 @@ struct ChainBatch {
     int* tr_acc = nullptr;
 };
 
+// ---- batched systems: replicas of proteins of uneven size (batch.hip) ----
+struct BProt {
+    int n;          // atoms
+    int excl_base;  // first of this protein's n + 1 entries in excl_off
+    int rot_off;    // first rotatable bond (run_off index)
+    int n_rot;
+    int dih_off;    // first dihedral record (dih_a..d, dih_toff index)
+    int n_dih;
+    int ss_off;
+    int n_ss;
+};
+struct BSys {
+    int prot;       // protein
+    int n;          // atoms
+    int aoff;       // first atom in the per-atom arrays
+    int coff;       // first cell counter (max_cells + 1 entries)
+    int max_cells;
+    int roff;       // first record (dihedrals then disulfides) in the flat record list
+};
+struct BState {
+    double e;           // energy of the current state
+    double e_new;       // energy of the last proposal
+    double temperature; // kT of the slot this system occupies
+    double delta, mu_f, u;
+    long long accepted, moves;
+    int step;           // MALA steps done
+    int bond;
+    int slot;
+    int status;         // 0 ok, 1 list capacity exceeded, 2 cell grid exceeded
+    int valid;          // the last proposal moved atoms
+    int flags;          // overflow bits of the current step
+};
+struct Batch {
+    int n_prot = 0, n_sys = 0, n_rep = 0, n_atoms = 0, cap = 0, n_rec = 0, cells = 0;
+    int lanes = 64;            // lanes per atom row in the pair kernel (autotuned)
+    int pair_block = 256;
+    double h = 0.02;           // MALA step, rad
+    uint32_t seed = 1;
+    int exchange_every = 0;
+    BProt* prot = nullptr;
+    BSys* sys = nullptr;
+    BState* st = nullptr;
+    Grid* grid = nullptr;
+    int* atom_sys = nullptr;
+    double* pos = nullptr;     // 4 per atom
+    double* posn = nullptr;    // 4 per atom (proposal)
+    float4* prm = nullptr;     // q, R*, eps, a
+    double* rstar = nullptr;
+    double* eps64 = nullptr;
+    double* a = nullptr;
+    double* born_sum = nullptr;
+    double* dEda = nullptr;
+    double* G = nullptr;       // 3 per atom, gradient of the current state
+    double* Gn = nullptr;      // 3 per atom, gradient of the proposal
+    double* part = nullptr;    // 5 per atom: DH, GB, LJ, HC, SASA
+    int* cell_of = nullptr;
+    int* cell_tmp = nullptr;
+    int* cell_atoms = nullptr;
+    int* cell_cnt = nullptr;
+    int* cell_start = nullptr;
+    uint32_t* nl = nullptr;    // n_atoms * cap, local j | excl bit
+    int* nl_cnt = nullptr;
+    int* flags = nullptr;      // [0] largest row
+    // topology, local atom indices, packed over proteins
+    int* excl_off = nullptr;
+    int* excl_idx = nullptr;
+    int* rb_i = nullptr;
+    int* rb_j = nullptr;
+    int* run_off = nullptr;
+    int2* side_run = nullptr;
+    int* dih_a = nullptr;
+    int* dih_b = nullptr;
+    int* dih_c = nullptr;
+    int* dih_d = nullptr;
+    int* dih_toff = nullptr;
+    double* dih_v2 = nullptr;
+    int* dih_n = nullptr;
+    double* dih_gamma = nullptr;
+    int* ss_i = nullptr;
+    int* ss_j = nullptr;
+    int* rec_sys = nullptr;
+    double* rec_e = nullptr;
+    double* terms = nullptr;   // 8 per system
+    int* slot_sys = nullptr;   // n_prot * n_rep
+    double* temps = nullptr;   // n_rep
+    long long* swaps = nullptr;  // 2 per protein
+    int trace_cap = 0;         // steps per system the trace holds (0 = off)
+    int trace_base = 0;        // step index of trace entry 0
+    int* tr_bond = nullptr;
+    double* tr_delta = nullptr;
+    double* tr_dE = nullptr;
+    int* tr_acc = nullptr;
+};
+
+// one evaluation of every system's proposal (init: the current states) and
+// the accept step; exchange after it when due
+hipError_t launch_batch_step(const Batch& b, int init, hipStream_t st);
+// evaluation kernels only, pair kernel variant chosen by lanes / block
+hipError_t launch_batch_pair(const Batch& b, int lanes, int block, hipStream_t st);
+
 hipError_t launch_chain_rebuild(ChainBatch& b, const DevSys& s, int nlist, int count, hipStream_t st);
 hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int nlist, hipStream_t st);
 
@@ -177,6 +277,16 @@ hipError_t launch_energy(DevSys& s, hipStream_t st);
 hipError_t launch_forces(DevSys& s, double* G, double* dEda, double* pre, double* dEdt, hipStream_t st);
 
 }  // namespace alma
+
+// host copy of a topology (owns the arrays behind an alma_topo view)
+struct HostTopo {
+    int n_atoms = 0, n_rot = 0, n_dih = 0, n_ss = 0, n_cp = 0, n_csp = 0;
+    std::vector<int32_t> excl_off, excl_idx, rb_i, rb_j, rb_kind, side_off, side_idx, dih_a, dih_b, dih_c,
+        dih_d, dih_toff, dih_n, ss_i, ss_j, cp_phi, cp_psi, csp_a, csp_b;
+    std::vector<double> rb_scale, dih_v2, dih_gamma;
+    void copy_from(const alma_topo* t);
+    alma_topo view() const;
+};
 
 struct alma_ctx {
     int device = 0;
@@ -194,6 +304,20 @@ struct alma_ctx {
     long long n_mc_launch = 0, n_rebuild_launch = 0;
     long long work[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // host copy of the loaded topology (the MALA path builds a batch from it)
+    HostTopo topo_copy;
+    bool has_topo_copy = false;
+    // batched replicas (parallel tempering / MALA)
+    alma::Batch batch;
+    std::vector<alma::BSys> h_bsys;
+    std::vector<alma::BProt> h_bprot;
+    int pt_step = 0;
+    bool pt_ready = false;
+    double t_pt_ms = 0.0;
+    long long pt_replays = 0;
+    std::vector<int> pt_tr_bond, pt_tr_acc;
+    std::vector<double> pt_tr_delta, pt_tr_dE;
+    int pt_tr_steps = 0;
     // forces buffers (sized for sys.n / n_rot when allocated)
     double *f_grad = nullptr, *f_dEda = nullptr, *f_pre = nullptr, *f_dEdt = nullptr;
     int f_n = -1;

@@ -1,0 +1,850 @@
+// batch.hip -- many systems in one launch: replicas of proteins of uneven
+// size, each evaluated exactly as calculate_potential does
+// (src/physics_engine.cpp:2242-2248: fresh neighbour list in its own cell
+// grid, HCT Born radii over the whole list, pair terms within 12 A, SASA
+// fold, dihedrals, disulfides) together with the gradient (Born chain rule,
+// SASA fold adjoint, dihedral reverse mode), and the gradient-guided (MALA)
+// torsion step and replica exchange built on them (no reference counterpart;
+// the exchange rule follows python/gui_main.py:1998-2012).
+//
+// Every kernel covers all systems: per-atom kernels run over the packed atom
+// range (one wavefront, or a lane group, per atom row), per-system kernels
+// run one workgroup per system.  No kernel allocates, copies through the
+// host or clears memory with a memset, so one step is a fixed kernel
+// sequence that a HIP graph captures once and replays.
+//
+// Random numbers: Philox4x32-10 (Salmon et al., SC'11), key {seed, system},
+// counter {step, 0, 0, 0}; exchanges key {seed, 0x80000000 | protein},
+// counter {round, slot, 1, 0}.
+#include "physics_dev.h"
+
+namespace alma {
+
+namespace {
+
+using namespace dev;
+
+constexpr int kSysBlock = 256;
+constexpr double kTwoM32 = 1.0 / 4294967296.0;
+
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                       uint32_t k1, uint32_t* out) {
+    uint32_t c[4] = {c0, c1, c2, c3};
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c[0];
+    out[1] = c[1];
+    out[2] = c[2];
+    out[3] = c[3];
+}
+
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// block-wide FP64 sum (kSysBlock threads), result valid in every thread
+__device__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    int w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane_id() == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < kSysBlock / kWave; ++k) t += sh[k];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ void cross3(const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+// rodrigues, src/physics_engine.cpp:1848-1862
+__device__ __forceinline__ void rotate(const double* p, const double* o, const double* u, double c, double sn,
+                                       double* q) {
+    double vx = p[0] - o[0], vy = p[1] - o[1], vz = p[2] - o[2];
+    double dot = u[0] * vx + u[1] * vy + u[2] * vz;
+    double cx = u[1] * vz - u[2] * vy, cy = u[2] * vx - u[0] * vz, cz = u[0] * vy - u[1] * vx;
+    double k = 1.0 - c;
+    q[0] = o[0] + vx * c + cx * sn + u[0] * dot * k;
+    q[1] = o[1] + vy * c + cy * sn + u[1] * dot * k;
+    q[2] = o[2] + vz * c + cz * sn + u[2] * dot * k;
+}
+
+// u . sum over the bond's side atoms of (x_i - o) x G_i  (block reduction)
+__device__ double side_torque(const Batch& b, const BSys& sy, const BProt& pr, int k, const double* x,
+                              const double* G, const double* o, const double* u, double* sh) {
+    double acc = 0.0;
+    for (int q = b.run_off[pr.rot_off + k]; q < b.run_off[pr.rot_off + k + 1]; ++q) {
+        int2 r = b.side_run[q];
+        for (int i = r.x + (int)threadIdx.x; i <= r.y; i += kSysBlock) {
+            double v[3] = {x[4 * i] - o[0], x[4 * i + 1] - o[1], x[4 * i + 2] - o[2]}, w[3];
+            cross3(v, G + 3 * i, w);
+            acc += u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+        }
+    }
+    return block_sum(acc, sh);
+}
+
+__device__ __forceinline__ bool bond_axis(const double* x, int bi, int bj, double* o, double* u) {
+    for (int d = 0; d < 3; ++d) {
+        o[d] = x[4 * bi + d];
+        u[d] = x[4 * bj + d] - o[d];
+    }
+    double l = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    if (l < 1e-10) return false;
+    for (int d = 0; d < 3; ++d) u[d] /= l;
+    return true;
+}
+
+// ── step 1: proposal, cell grid, clears ─────────────────────────────────────
+__global__ void __launch_bounds__(kSysBlock) kb_propose(Batch b, int init) {
+    const int s = blockIdx.x;
+    const BSys sy = b.sys[s];
+    const BProt pr = b.prot[sy.prot];
+    BState& st = b.st[s];
+    const double* x = b.pos + 4 * (size_t)sy.aoff;
+    double* xn = b.posn + 4 * (size_t)sy.aoff;
+    const double* G = b.G + 3 * (size_t)sy.aoff;
+    __shared__ double sh[kSysBlock / kWave];
+    __shared__ double so[3], su[3], sc, ssn;
+    __shared__ int sk, svalid;
+    __shared__ double sxi, su_draw;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        svalid = 0;
+        sk = -1;
+        if (!init && st.status == 0 && pr.n_rot > 0) {
+            uint32_t w[4];
+            philox((uint32_t)st.step, 0u, 0u, 0u, b.seed, (uint32_t)s, w);
+            int k = (int)(((uint64_t)w[0] * (uint64_t)pr.n_rot) >> 32);
+            sk = k;
+            sxi = sqrt(-2.0 * log(((double)w[1] + 1.0) * kTwoM32)) * cos(2.0 * M_PI * ((double)w[2] * kTwoM32));
+            su_draw = ((double)w[3] + 0.5) * kTwoM32;
+            double o[3], u[3];
+            int bi = b.rb_i[pr.rot_off + k], bj = b.rb_j[pr.rot_off + k];
+            bool has_side = b.run_off[pr.rot_off + k + 1] > b.run_off[pr.rot_off + k];
+            if (has_side && bond_axis(x, bi, bj, o, u)) {
+                svalid = 1;
+                for (int d = 0; d < 3; ++d) {
+                    so[d] = o[d];
+                    su[d] = u[d];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int k = sk;
+    if (svalid) {
+        double o[3] = {so[0], so[1], so[2]}, u[3] = {su[0], su[1], su[2]};
+        double g = side_torque(b, sy, pr, k, x, G, o, u, sh);
+        if (t == 0) {
+            double beta = 1.0 / st.temperature;
+            double mu = -0.5 * b.h * b.h * beta * g;
+            double delta = mu + b.h * sxi;
+            st.mu_f = mu;
+            st.delta = delta;
+            st.u = su_draw;
+            sc = cos(delta);
+            ssn = sin(delta);
+        }
+    } else if (t == 0) {
+        st.delta = 0.0;
+    }
+    if (t == 0) {
+        st.bond = k;
+        st.valid = svalid;
+        st.flags = 0;
+    }
+    __syncthreads();
+    // copy, then rotate the side
+    for (int i = t; i < sy.n; i += kSysBlock) {
+        xn[4 * i] = x[4 * i];
+        xn[4 * i + 1] = x[4 * i + 1];
+        xn[4 * i + 2] = x[4 * i + 2];
+        xn[4 * i + 3] = 0.0;
+    }
+    __syncthreads();
+    if (svalid) {
+        double o[3] = {so[0], so[1], so[2]}, u[3] = {su[0], su[1], su[2]};
+        for (int q = b.run_off[pr.rot_off + k]; q < b.run_off[pr.rot_off + k + 1]; ++q) {
+            int2 r = b.side_run[q];
+            for (int i = r.x + t; i <= r.y; i += kSysBlock) rotate(x + 4 * i, o, u, sc, ssn, xn + 4 * i);
+        }
+    }
+    __syncthreads();
+    // cell grid of the proposal (NeighborList::build, src/physics_engine.cpp:293-342)
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = xn[d];
+    for (int i = t; i < sy.n; i += kSysBlock)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], xn[4 * i + d]);
+            hi[d] = fmax(hi[d], xn[4 * i + d]);
+        }
+    __shared__ double slo[3][kSysBlock], shi[3][kSysBlock];
+    for (int d = 0; d < 3; ++d) {
+        slo[d][t] = lo[d];
+        shi[d][t] = hi[d];
+    }
+    __syncthreads();
+    for (int w = kSysBlock / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int d = 0; d < 3; ++d) {
+                slo[d][t] = fmin(slo[d][t], slo[d][t + w]);
+                shi[d][t] = fmax(shi[d][t], shi[d][t + w]);
+            }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma clang fp contract(off)
+        Grid& g = b.grid[s];
+        const double cs = sqrt(kNlRcut2) + 1e-6;
+        g.cs = cs;
+        int ncell = 1;
+        for (int d = 0; d < 3; ++d) {
+            double l = slo[d][0] - cs;
+            double len = shi[d][0] - l + 2.0 * cs;
+            int c = (int)ceil(len / cs);
+            g.lo[d] = l;
+            g.nc[d] = c < 1 ? 1 : c;
+            ncell *= g.nc[d];
+        }
+        g.ncell = ncell;
+        if (ncell > sy.max_cells) st.flags |= 2;
+    }
+    // clears for the evaluation
+    for (int c = t; c <= sy.max_cells; c += kSysBlock) b.cell_cnt[sy.coff + c] = 0;
+    for (int i = t; i < sy.n; i += kSysBlock) {
+        size_t gi = (size_t)sy.aoff + i;
+        b.born_sum[gi] = 0.0;
+        b.dEda[gi] = 0.0;
+        b.Gn[3 * gi] = 0.0;
+        b.Gn[3 * gi + 1] = 0.0;
+        b.Gn[3 * gi + 2] = 0.0;
+    }
+}
+
+// ── neighbour list per system ───────────────────────────────────────────────
+__global__ void kb_assign(Batch b) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.n_atoms) return;
+    int s = b.atom_sys[g];
+    const BSys& sy = b.sys[s];
+    const Grid& gr = b.grid[s];
+    int id = 0;
+    if (gr.ncell <= sy.max_cells) {
+        int c[3];
+        for (int d = 0; d < 3; ++d) {
+            int v = (int)floor((b.posn[4 * (size_t)g + d] - gr.lo[d]) / gr.cs);
+            c[d] = min(gr.nc[d] - 1, max(0, v));
+        }
+        id = c[0] + gr.nc[0] * (c[1] + gr.nc[1] * c[2]);
+    }
+    b.cell_of[g] = id;
+    atomicAdd(&b.cell_cnt[sy.coff + id], 1);
+}
+
+__global__ void __launch_bounds__(1024) kb_scan(Batch b) {
+    const int s = blockIdx.x;
+    const BSys sy = b.sys[s];
+    __shared__ int buf[1024];
+    int ncell = min(b.grid[s].ncell, sy.max_cells);
+    int* cnt = b.cell_cnt + sy.coff;
+    int* start = b.cell_start + sy.coff;
+    int t = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < ncell; base += 1024) {
+        int v = (base + t < ncell) ? cnt[base + t] : 0;
+        buf[t] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int add = t >= o ? buf[t - o] : 0;
+            __syncthreads();
+            buf[t] += add;
+            __syncthreads();
+        }
+        if (base + t < ncell) start[base + t] = carry + buf[t] - v;
+        int tot = buf[1023];
+        __syncthreads();
+        carry += tot;
+    }
+    if (t == 0) start[ncell] = carry;
+    for (int k = t; k < ncell; k += 1024) cnt[k] = 0;
+}
+
+__global__ void kb_place(Batch b) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.n_atoms) return;
+    const BSys& sy = b.sys[b.atom_sys[g]];
+    int c = b.cell_of[g];
+    int i = g - sy.aoff;
+    b.cell_tmp[sy.aoff + b.cell_start[sy.coff + c] + atomicAdd(&b.cell_cnt[sy.coff + c], 1)] = i;
+}
+
+__global__ void kb_rank(Batch b) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.n_atoms) return;
+    const BSys& sy = b.sys[b.atom_sys[g]];
+    int c = b.cell_of[g];
+    int i = g - sy.aoff;
+    int lo = b.cell_start[sy.coff + c], hi = b.cell_start[sy.coff + c + 1];
+    const int* tmp = b.cell_tmp + sy.aoff;
+    int r = 0;
+    for (int k = lo; k < hi; ++k) r += tmp[k] < i;
+    b.cell_atoms[sy.aoff + lo + r] = i;
+}
+
+// one wavefront per atom: 27 cells dz -> dy -> dx, members ascending, j > i,
+// d^2 < 196; ballot prefix keeps the scan order
+__global__ void __launch_bounds__(kBlock) kb_nl(Batch b) {
+    int g = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (g >= b.n_atoms) return;
+    const int s = b.atom_sys[g];
+    const BSys& sy = b.sys[s];
+    const Grid& gr = b.grid[s];
+    const int lane = lane_id();
+    if (gr.ncell > sy.max_cells) {   // grid too large: empty rows, the step is flagged
+        if (lane == 0) b.nl_cnt[g] = 0;
+        return;
+    }
+    const BProt& pr = b.prot[sy.prot];
+    const int i = g - sy.aoff;
+    const double* x = b.posn + 4 * (size_t)sy.aoff;
+    const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
+    const int* start = b.cell_start + sy.coff;
+    const int* atoms = b.cell_atoms + sy.aoff;
+    int c = b.cell_of[g];
+    int cx = c % gr.nc[0], cy = (c / gr.nc[0]) % gr.nc[1], cz = c / (gr.nc[0] * gr.nc[1]);
+    int eb = b.excl_off[pr.excl_base + i], ee = b.excl_off[pr.excl_base + i + 1];
+    uint32_t* row = b.nl + (size_t)g * b.cap;
+    int count = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        if (z < 0 || z >= gr.nc[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            if (y < 0 || y >= gr.nc[1]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                int xx = cx + dx;
+                if (xx < 0 || xx >= gr.nc[0]) continue;
+                int cc = xx + gr.nc[0] * (y + gr.nc[1] * z);
+                int lo = start[cc], hi = start[cc + 1];
+                for (int base = lo; base < hi; base += kWave) {
+                    int k = base + lane;
+                    bool keep = false;
+                    int j = 0;
+                    if (k < hi) {
+                        j = atoms[k];
+                        if (j > i) keep = dist2_exact(pi, x + 4 * j) < kNlRcut2;
+                    }
+                    uint64_t m = __ballot(keep);
+                    if (keep) {
+                        int slot = count + __popcll(m & ((1ull << lane) - 1ull));
+                        uint32_t v = (uint32_t)j;
+                        if (in_sorted(b.excl_idx, eb, ee, j)) v |= kExclBit;
+                        if (slot < b.cap) row[slot] = v;
+                    }
+                    count += __popcll(m);
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        b.nl_cnt[g] = min(count, b.cap);
+        atomicMax(&b.flags[0], count);
+        if (count > b.cap) atomicOr(&b.st[s].flags, 1);
+    }
+}
+
+// ── Born radii (born_radii :2024-2039, both directions, every entry) ───────
+__global__ void __launch_bounds__(kBlock) kb_born(Batch b) {
+    int g = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (g >= b.n_atoms) return;
+    const BSys& sy = b.sys[b.atom_sys[g]];
+    const int lane = lane_id();
+    const int i = g - sy.aoff;
+    const double* x = b.posn + 4 * (size_t)sy.aoff;
+    const float4* prm = b.prm + sy.aoff;
+    double* sum = b.born_sum + sy.aoff;
+    const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
+    float ri = prm[i].y;
+    int m = b.nl_cnt[g];
+    const uint32_t* row = b.nl + (size_t)g * b.cap;
+    double acc = 0.0;
+    for (int k = lane; k < m; k += kWave) {
+        int j = (int)(row[k] & kIdxMask);
+        double r2d = dist2_exact(pi, x + 4 * j);
+        float r2 = (float)r2d, r = sqrtf(r2);
+        float rj = prm[j].y;
+        acc += (double)hct_f(r, r2, ri, rj);
+        atomicAdd(&sum[j], (double)hct_f(r, r2, rj, ri));
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) atomicAdd(&sum[i], acc);
+}
+
+__global__ void kb_born_fin(Batch b) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.n_atoms) return;
+    double v = 1.0 / fmax(1.0 / b.rstar[g] - 0.5 * b.born_sum[g], 2.0);
+    b.a[g] = v;
+    b.prm[g].w = (float)v;
+}
+
+// ── pair terms and their radial / Born-radius derivatives, L lanes per row ──
+template <int L>
+__global__ void kb_pair(Batch b) {
+    const int groups_per_block = blockDim.x / L;
+    int g = blockIdx.x * groups_per_block + (threadIdx.x / L);
+    bool live = g < b.n_atoms;
+    if (!live) g = b.n_atoms - 1;   // keep the lane group converged for the shuffles
+    const BSys& sy = b.sys[b.atom_sys[g]];
+    const int sub = threadIdx.x % L;
+    const int i = g - sy.aoff;
+    const double* x = b.posn + 4 * (size_t)sy.aoff;
+    const float4* prm = b.prm + sy.aoff;
+    const double* Rs = b.rstar + sy.aoff;
+    const double* Es = b.eps64 + sy.aoff;
+    double* G = b.Gn + 3 * (size_t)sy.aoff;
+    double* dEda = b.dEda + sy.aoff;
+    const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
+    const float4 qi = prm[i];
+    const double Ri = Rs[i], Ei = Es[i];
+    int m = live ? b.nl_cnt[g] : 0;
+    const uint32_t* row = b.nl + (size_t)g * b.cap;
+    double dh = 0, gb = 0, lj = 0, hc = 0, gi0 = 0, gi1 = 0, gi2 = 0, dai = 0;
+    for (int k = sub; k < m; k += L) {
+        uint32_t e = row[k];
+        if (e & kExclBit) continue;
+        int j = (int)(e & kIdxMask);
+        double r2 = dist2_exact(pi, x + 4 * j);
+        if (r2 > kPairCut2) continue;
+        const float4 qj = prm[j];
+        const double Rj = Rs[j], Ej = Es[j];
+        PairTerms pt = pair_terms(r2, qi, qj, Ri, Rj, Ei, Ej);
+        dh += pt.dh;
+        gb += pt.gb;
+        lj += pt.lj;
+        hc += pt.hc;
+        // derivatives (FP32 DH / GB, FP64 LJ / hard core)
+        float r2f = (float)r2, r = sqrt_f(r2f), ir = rcp_f(r);
+        float qp = qi.x * qj.x;
+        float edh = (float)(kCoulomb / kEpsWater) * qp * ir * exp_f(-(float)kKappa * r);
+        float ddh = edh * (-ir - (float)kKappa);
+        float D = qi.w * qj.w;
+        float xx = r2f * rcp_f(4.0f * D);
+        float gg = xx < 87.0f ? exp_f(-xx) : 0.0f;
+        float f = sqrt_f(r2f + D * gg), iff = rcp_f(f);
+        float egb = (float)kGbCoef * qp * iff;
+        float dgb = -egb * iff * (2.0f * r - 0.5f * r * gg) * 0.5f * iff;
+        float dD = -egb * iff * gg * (1.0f + xx) * 0.5f * iff;
+        double sig = Ri + Rj;
+        double rr = sqrt(r2);
+        double s2 = sig * sig / r2, s6 = s2 * s2 * s2;
+        double dlj = 4.0 * sqrt(Ei * Ej) * (-12.0 * s6 * s6 + 6.0 * s6) / rr;
+        double dhc = 0.0, rc = sig * kHardFrac;
+        if (r2 < rc * rc) {
+            double c2 = rc * rc / r2, c6 = c2 * c2 * c2, p12 = c6 * c6, xc = p12 - 1.0;
+            if (kHardScale * xc * xc < kHardCap) dhc = kHardScale * 2.0 * xc * (-12.0 * p12 / rr);
+        }
+        double dr = ((double)ddh + (double)dgb + dlj + dhc) / rr;
+        double u0 = (pi[0] - x[4 * j]) * dr, u1 = (pi[1] - x[4 * j + 1]) * dr, u2 = (pi[2] - x[4 * j + 2]) * dr;
+        gi0 += u0;
+        gi1 += u1;
+        gi2 += u2;
+        atomicAdd(&G[3 * j], -u0);
+        atomicAdd(&G[3 * j + 1], -u1);
+        atomicAdd(&G[3 * j + 2], -u2);
+        dai += (double)dD * (double)qj.w;
+        atomicAdd(&dEda[j], (double)dD * (double)qi.w);
+    }
+    dh = group_sum<L>(dh);
+    gb = group_sum<L>(gb);
+    lj = group_sum<L>(lj);
+    hc = group_sum<L>(hc);
+    gi0 = group_sum<L>(gi0);
+    gi1 = group_sum<L>(gi1);
+    gi2 = group_sum<L>(gi2);
+    dai = group_sum<L>(dai);
+    if (live && sub == 0) {
+        double* part = b.part + 5 * (size_t)g;
+        part[0] = dh;
+        part[1] = gb;
+        part[2] = lj;
+        part[3] = hc;
+        atomicAdd(&G[3 * i], gi0);
+        atomicAdd(&G[3 * i + 1], gi1);
+        atomicAdd(&G[3 * i + 2], gi2);
+        atomicAdd(&dEda[i], dai);
+    }
+}
+
+// ── SASA (sasa_nonpolar :2068-2083) and the adjoint of its ordered fold ─────
+__global__ void kb_sasa(Batch b) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.n_atoms) return;
+    const BSys& sy = b.sys[b.atom_sys[g]];
+    const int i = g - sy.aoff;
+    const double* x = b.posn + 4 * (size_t)sy.aoff;
+    const float4* prm = b.prm + sy.aoff;
+    double* G = b.Gn + 3 * (size_t)sy.aoff;
+    const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
+    const double ri = (double)prm[i].y + kProbe;
+    const double k2 = 2.0 * M_PI * ri * ri / (2.0 * ri);
+    const int m = b.nl_cnt[g];
+    const uint32_t* row = b.nl + (size_t)g * b.cap;
+    double sa = 4.0 * M_PI * ri * ri;
+    int clamped = 0;
+    for (int k = 0; k < m; ++k) {
+        int j = (int)(row[k] & kIdxMask);
+        double dc = ri + (double)prm[j].y + kProbe;
+        double r2 = dist2_exact(pi, x + 4 * j);
+        if (r2 >= dc * dc) continue;
+        double r = (double)sqrtf((float)r2);
+        if (r >= dc) continue;
+        double cap = k2 * (dc - r);
+        if (sa * 0.85 < cap) {
+            sa -= sa * 0.85;
+            ++clamped;
+        } else {
+            sa -= cap;
+        }
+    }
+    b.part[5 * (size_t)g + 4] = kGammaSA * fmax(0.0, sa);
+    if (!(sa > 0.0)) return;
+    double g0 = 0, g1 = 0, g2 = 0;
+    sa = 4.0 * M_PI * ri * ri;
+    int before = 0;
+    for (int k = 0; k < m; ++k) {
+        int j = (int)(row[k] & kIdxMask);
+        double dc = ri + (double)prm[j].y + kProbe;
+        double r2 = dist2_exact(pi, x + 4 * j);
+        if (r2 >= dc * dc) continue;
+        double r = (double)sqrtf((float)r2);
+        if (r >= dc) continue;
+        double cap = k2 * (dc - r);
+        if (sa * 0.85 < cap) {
+            sa -= sa * 0.85;
+            ++before;
+            continue;
+        }
+        sa -= cap;
+        double sc = kGammaSA * k2 * pow(0.15, (double)(clamped - before)) / sqrt(r2);
+        double u0 = (pi[0] - x[4 * j]) * sc, u1 = (pi[1] - x[4 * j + 1]) * sc, u2 = (pi[2] - x[4 * j + 2]) * sc;
+        g0 += u0;
+        g1 += u1;
+        g2 += u2;
+        atomicAdd(&G[3 * j], -u0);
+        atomicAdd(&G[3 * j + 1], -u1);
+        atomicAdd(&G[3 * j + 2], -u2);
+    }
+    atomicAdd(&G[3 * i], g0);
+    atomicAdd(&G[3 * i + 1], g1);
+    atomicAdd(&G[3 * i + 2], g2);
+}
+
+// d hct(r; ri, rj) / dr
+__device__ __forceinline__ float dhct(float r, float ri, float rj) {
+    float U = r + rj;
+    if (ri >= U) return 0.0f;
+    float d = r - rj;
+    float L = fmaxf(fabsf(d), ri);
+    float dL = fabsf(d) > ri ? (d > 0.0f ? 1.0f : -1.0f) : 0.0f;
+    float c = ri * ri - rj * rj;
+    float ir = rcp_f(r), iri2 = rcp_f(ri * ri);
+    float B = (r * r + c) * 0.25f * iri2 * ir * ir;
+    float dB = -0.5f * c * iri2 * ir * ir * ir;
+    float lg = log_f(L * rcp_f(U));
+    float iL = rcp_f(L), iU = rcp_f(U);
+    return -dL * iL * iL + iU * iU + dB * lg + B * (dL * iL - iU);
+}
+
+// Born chain rule: a = 1/(1/R* - S/2) while a < 1/2, da/dS = a^2/2
+__global__ void __launch_bounds__(kBlock) kb_bornchain(Batch b) {
+    int g = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (g >= b.n_atoms) return;
+    const BSys& sy = b.sys[b.atom_sys[g]];
+    const int lane = lane_id();
+    const int i = g - sy.aoff;
+    const double* x = b.posn + 4 * (size_t)sy.aoff;
+    const float4* prm = b.prm + sy.aoff;
+    const double* A = b.a + sy.aoff;
+    const double* dEda = b.dEda + sy.aoff;
+    double* G = b.Gn + 3 * (size_t)sy.aoff;
+    double ai = A[i];
+    double wi = ai < 0.5 ? dEda[i] * 0.5 * ai * ai : 0.0;
+    const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
+    float ri = prm[i].y;
+    int m = b.nl_cnt[g];
+    const uint32_t* row = b.nl + (size_t)g * b.cap;
+    double g0 = 0, g1 = 0, g2 = 0;
+    for (int k = lane; k < m; k += kWave) {
+        int j = (int)(row[k] & kIdxMask);
+        double aj = A[j];
+        double wj = aj < 0.5 ? dEda[j] * 0.5 * aj * aj : 0.0;
+        if (wi == 0.0 && wj == 0.0) continue;
+        double r2 = dist2_exact(pi, x + 4 * j), r = sqrt(r2);
+        float rf = (float)r, rj = prm[j].y;
+        double d = (wi * (double)dhct(rf, ri, rj) + wj * (double)dhct(rf, rj, ri)) / r;
+        double u0 = (pi[0] - x[4 * j]) * d, u1 = (pi[1] - x[4 * j + 1]) * d, u2 = (pi[2] - x[4 * j + 2]) * d;
+        g0 += u0;
+        g1 += u1;
+        g2 += u2;
+        atomicAdd(&G[3 * j], -u0);
+        atomicAdd(&G[3 * j + 1], -u1);
+        atomicAdd(&G[3 * j + 2], -u2);
+    }
+    g0 = wave_sum(g0);
+    g1 = wave_sum(g1);
+    g2 = wave_sum(g2);
+    if (lane == 0) {
+        atomicAdd(&G[3 * i], g0);
+        atomicAdd(&G[3 * i + 1], g1);
+        atomicAdd(&G[3 * i + 2], g2);
+    }
+}
+
+// ── dihedral records and disulfides: energy and gradient (FP64) ────────────
+__global__ void kb_bonded(Batch b) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= b.n_rec) return;
+    const int s = b.rec_sys[r];
+    const BSys& sy = b.sys[s];
+    const BProt& pr = b.prot[sy.prot];
+    const double* x = b.posn + 4 * (size_t)sy.aoff;
+    double* G = b.Gn + 3 * (size_t)sy.aoff;
+    int k = r - sy.roff;
+    if (k < pr.n_dih) {
+        k += pr.dih_off;
+        int ia = b.dih_a[k], ib = b.dih_b[k], ic = b.dih_c[k], id = b.dih_d[k];
+        double b1[3], b2[3], b3[3], n1[3], n2[3], m1[3];
+        for (int q = 0; q < 3; ++q) {
+            b1[q] = x[4 * ia + q] - x[4 * ib + q];
+            b2[q] = x[4 * ic + q] - x[4 * ib + q];
+            b3[q] = x[4 * id + q] - x[4 * ic + q];
+        }
+        cross3(b1, b2, n1);
+        cross3(b2, b3, n2);
+        cross3(n1, b2, m1);
+        double X = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+        double Y = m1[0] * n2[0] + m1[1] * n2[1] + m1[2] * n2[2];
+        double phi = dih_angle_d(x, ia, ib, ic, id);
+        double e = 0.0, gphi = 0.0;
+        for (int m = b.dih_toff[k]; m < b.dih_toff[k + 1]; ++m) {
+            double arg = (double)b.dih_n[m] * phi - b.dih_gamma[m];
+            e += b.dih_v2[m] * (1.0 + cos(arg));
+            gphi += -b.dih_v2[m] * b.dih_n[m] * sin(arg);
+        }
+        b.rec_e[r] = e;
+        double den = X * X + Y * Y;
+        if (den == 0.0) return;
+        double gX = -Y / den * gphi, gY = X / den * gphi;
+        double gn1[3], gn2[3], gm1[3], gb1[3] = {0, 0, 0}, gb2[3] = {0, 0, 0}, gb3[3] = {0, 0, 0}, tt[3];
+        for (int q = 0; q < 3; ++q) {
+            gn1[q] = gX * n2[q];
+            gn2[q] = gX * n1[q] + gY * m1[q];
+            gm1[q] = gY * n2[q];
+        }
+        cross3(b2, gm1, tt);
+        for (int q = 0; q < 3; ++q) gn1[q] += tt[q];
+        cross3(gm1, n1, tt);
+        for (int q = 0; q < 3; ++q) gb2[q] += tt[q];
+        cross3(b3, gn2, tt);
+        for (int q = 0; q < 3; ++q) gb2[q] += tt[q];
+        cross3(gn2, b2, tt);
+        for (int q = 0; q < 3; ++q) gb3[q] += tt[q];
+        cross3(b2, gn1, tt);
+        for (int q = 0; q < 3; ++q) gb1[q] += tt[q];
+        cross3(gn1, b1, tt);
+        for (int q = 0; q < 3; ++q) gb2[q] += tt[q];
+        for (int q = 0; q < 3; ++q) {
+            atomicAdd(&G[3 * ia + q], gb1[q]);
+            atomicAdd(&G[3 * ib + q], -gb1[q] - gb2[q]);
+            atomicAdd(&G[3 * ic + q], gb2[q] - gb3[q]);
+            atomicAdd(&G[3 * id + q], gb3[q]);
+        }
+    } else {
+        k = k - pr.n_dih + pr.ss_off;
+        int i = b.ss_i[k], j = b.ss_j[k];
+        double dx = x[4 * i] - x[4 * j], dy = x[4 * i + 1] - x[4 * j + 1], dz = x[4 * i + 2] - x[4 * j + 2];
+        double rr = sqrt(dx * dx + dy * dy + dz * dz);
+        double dr = rr - kR0ss;
+        b.rec_e[r] = kKss * dr * dr;
+        double sc = 2.0 * kKss * dr / rr;
+        double v[3] = {sc * dx, sc * dy, sc * dz};
+        for (int q = 0; q < 3; ++q) {
+            atomicAdd(&G[3 * i + q], v[q]);
+            atomicAdd(&G[3 * j + q], -v[q]);
+        }
+    }
+}
+
+// ── per-system totals, Metropolis-Hastings decision, commit ────────────────
+__global__ void __launch_bounds__(kSysBlock) kb_accept(Batch b, int init) {
+    const int s = blockIdx.x;
+    const BSys sy = b.sys[s];
+    const BProt pr = b.prot[sy.prot];
+    BState& st = b.st[s];
+    __shared__ double sh[kSysBlock / kWave];
+    __shared__ int sacc;
+    const int t = threadIdx.x;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = t; i < sy.n; i += kSysBlock) {
+        const double* p = b.part + 5 * ((size_t)sy.aoff + i);
+        for (int q = 0; q < 5; ++q) v[q] += p[q];
+    }
+    const int nrec = pr.n_dih + pr.n_ss;
+    for (int k = t; k < nrec; k += kSysBlock) v[k < pr.n_dih ? 5 : 6] += b.rec_e[sy.roff + k];
+    double tot[7];
+    for (int q = 0; q < 7; ++q) tot[q] = block_sum(v[q], sh);
+    const double sasa = kBetaSA + tot[4];
+    const double E = sasa + tot[5] + tot[6] + tot[0] + tot[1] + tot[2] + tot[3];
+    // proposal torque at the new state
+    double gn = 0.0;
+    const bool valid = st.valid != 0;
+    if (valid && !init) {
+        const double* xn = b.posn + 4 * (size_t)sy.aoff;
+        const double* Gn = b.Gn + 3 * (size_t)sy.aoff;
+        double o[3], u[3];
+        int k = st.bond;
+        bond_axis(xn, b.rb_i[pr.rot_off + k], b.rb_j[pr.rot_off + k], o, u);
+        gn = side_torque(b, sy, pr, k, xn, Gn, o, u, sh);
+    }
+    if (t == 0) {
+        double* tm = b.terms + 8 * (size_t)s;
+        tm[0] = tot[0];
+        tm[1] = tot[1];
+        tm[2] = tot[2];
+        tm[3] = tot[3];
+        tm[4] = sasa;
+        tm[5] = tot[5];
+        tm[6] = tot[6];
+        tm[7] = E;
+        st.e_new = E;
+        int ovf = st.flags & 3;
+        bool acc = false;
+        double dE = 0.0;
+        if (ovf) {
+            st.status = (ovf & 2) ? 2 : 1;
+        } else if (init) {
+            acc = true;
+        } else if (st.status == 0 && valid) {
+            double beta = 1.0 / st.temperature;
+            double h2 = b.h * b.h;
+            double mu_r = -0.5 * h2 * beta * gn;
+            dE = E - st.e;
+            double lf = (st.delta - st.mu_f) * (st.delta - st.mu_f);
+            double lr = (-st.delta - mu_r) * (-st.delta - mu_r);
+            double log_alpha = -beta * dE - (lr - lf) / (2.0 * h2);
+            acc = log(st.u) < log_alpha;
+            st.moves += 1;
+            if (acc) st.accepted += 1;
+        }
+        if (!init && st.status == 0) {
+            int ts = st.step - b.trace_base;
+            if (ts >= 0 && ts < b.trace_cap) {
+                size_t o = (size_t)s * b.trace_cap + ts;
+                b.tr_bond[o] = st.bond;
+                b.tr_delta[o] = st.delta;
+                b.tr_dE[o] = valid ? E - st.e : 0.0;
+                b.tr_acc[o] = acc ? 1 : 0;
+            }
+            st.step += 1;
+        }
+        if (acc) st.e = E;
+        sacc = acc ? 1 : 0;
+    }
+    __syncthreads();
+    if (sacc) {
+        double* x = b.pos + 4 * (size_t)sy.aoff;
+        const double* xn = b.posn + 4 * (size_t)sy.aoff;
+        double* G = b.G + 3 * (size_t)sy.aoff;
+        const double* Gn = b.Gn + 3 * (size_t)sy.aoff;
+        for (int i = t; i < sy.n; i += kSysBlock) {
+            x[4 * i] = xn[4 * i];
+            x[4 * i + 1] = xn[4 * i + 1];
+            x[4 * i + 2] = xn[4 * i + 2];
+        }
+        for (int i = t; i < 3 * sy.n; i += kSysBlock) G[i] = Gn[i];
+    }
+}
+
+// ── replica exchange (python/gui_main.py:1998-2012), one thread per protein ──
+__global__ void kb_exchange(Batch b) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= b.n_prot || b.exchange_every <= 0) return;
+    const int R = b.n_rep;
+    int* slot_sys = b.slot_sys + (size_t)p * R;
+    const int step = b.st[slot_sys[0]].step;
+    if (step == 0 || step % b.exchange_every != 0) return;
+    for (int a = 0; a < R; ++a)
+        if (b.st[slot_sys[a]].status != 0) return;
+    const int e = step / b.exchange_every - 1;
+    for (int a = e % 2; a + 1 < R; a += 2) {
+        int sa = slot_sys[a], sb = slot_sys[a + 1];
+        double xv = (b.st[sa].e - b.st[sb].e) * (1.0 / b.temps[a] - 1.0 / b.temps[a + 1]);
+        uint32_t w[4];
+        philox((uint32_t)e, (uint32_t)a, 1u, 0u, b.seed, 0x80000000u | (uint32_t)p, w);
+        double v = ((double)w[0] + 0.5) * kTwoM32;
+        b.swaps[2 * p] += 1;
+        if (xv >= 0.0 || v < exp(xv)) {
+            b.swaps[2 * p + 1] += 1;
+            slot_sys[a] = sb;
+            slot_sys[a + 1] = sa;
+            b.st[sb].slot = a;
+            b.st[sa].slot = a + 1;
+            b.st[sb].temperature = b.temps[a];
+            b.st[sa].temperature = b.temps[a + 1];
+        }
+    }
+}
+
+inline unsigned blocks_for(long long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+hipError_t launch_batch_pair(const Batch& b, int lanes, int block, hipStream_t st) {
+    unsigned grid = blocks_for(b.n_atoms, block / lanes);
+    switch (lanes) {
+        case 16: kb_pair<16><<<grid, block, 0, st>>>(b); break;
+        case 32: kb_pair<32><<<grid, block, 0, st>>>(b); break;
+        default: kb_pair<64><<<grid, block, 0, st>>>(b); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step(const Batch& b, int init, hipStream_t st) {
+    const int N = b.n_atoms;
+    kb_propose<<<b.n_sys, kSysBlock, 0, st>>>(b, init);
+    kb_assign<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
+    kb_scan<<<b.n_sys, 1024, 0, st>>>(b);
+    kb_place<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
+    kb_rank<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
+    kb_nl<<<blocks_for(N, kWavesPerBlock), kBlock, 0, st>>>(b);
+    kb_born<<<blocks_for(N, kWavesPerBlock), kBlock, 0, st>>>(b);
+    kb_born_fin<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
+    hipError_t e = launch_batch_pair(b, b.lanes, b.pair_block, st);
+    if (e != hipSuccess) return e;
+    kb_sasa<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
+    kb_bornchain<<<blocks_for(N, kWavesPerBlock), kBlock, 0, st>>>(b);
+    if (b.n_rec > 0) kb_bonded<<<blocks_for(b.n_rec, kBlock), kBlock, 0, st>>>(b);
+    kb_accept<<<b.n_sys, kSysBlock, 0, st>>>(b, init);
+    if (!init) kb_exchange<<<blocks_for(b.n_prot, 64), 64, 0, st>>>(b);
+    return hipGetLastError();
+}
+
+}  // namespace alma

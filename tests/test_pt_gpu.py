@@ -1,0 +1,115 @@
+"""Batched replicas on the GPU (proteins of uneven size, several temperature
+slots each) against the FP64 oracle:
+  * the batched evaluation equals calculate_potential per system and term;
+  * MALA torsion steps and replica exchanges follow the oracle's trajectory:
+    same bonds, proposals within FP32 gradient error, same accept and swap
+    decisions, energies and final coordinates within tolerance;
+  * alma_mc_run in MALA mode is the same sampler with one protein.
+Tolerances: per term 1e-5 relative; a proposal's delta within 1e-6 rad (its
+drift term carries the FP32 gradient error, h^2/2 * dE/dtheta / kT);
+step dE within 1e-5 of (sum of |terms| + |dE|); coordinates within 1e-5 A."""
+import numpy as np
+import pytest
+
+import systems as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KT = 0.59616
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from alma import _native
+    c = _native.Context(0)
+    yield c
+    c.close()
+
+
+def _proteins():
+    ps = [S.chain(20, 4), S.chain(35, 6), S.chain_ss()]
+    cs = []
+    for s in ps:
+        pairs = [tuple(int(v) for v in p) for p in s["ss_pairs"]]
+        cs.append(S.csr(S.topology(s, pairs)))
+    return ps, cs
+
+
+def _l1(terms):
+    return float(np.sum(np.abs(np.asarray(terms)[:7])))
+
+
+def test_batched_energy_matches_oracle(ctx):
+    ps, cs = _proteins()
+    R = 2
+    xyz = [p["xyz"] for p in ps for _ in range(R)]
+    ctx.pt_set(ps, cs, R, [KT, 2 * KT], xyz)
+    got = ctx.pt_energy()
+    for k in range(len(ps) * R):
+        ref = O.energy_terms(ps[k // R], cs[k // R])
+        for q, name in enumerate(O.TERMS):
+            if ref[name] == 0.0:
+                assert got[k, q] == 0.0, (k, name)
+                continue
+            assert abs(got[k, q] - ref[name]) <= 1e-5 * abs(ref[name]), (k, name, got[k, q], ref[name])
+
+
+@pytest.mark.parametrize("h,exchange", [(0.02, 0), (0.12, 5)])
+def test_mala_pt_trajectory_matches_oracle(ctx, h, exchange):
+    ps, cs = _proteins()
+    R = 2
+    temps = [KT, 1.6 * KT]
+    xyz = [p["xyz"] for p in ps for _ in range(R)]
+    steps = 30
+    ctx.pt_set(ps, cs, R, temps, xyz)
+    g = ctx.pt_run(steps, exchange_every=exchange, mala_step=h, seed=17, trace=True)
+    o = O.pt_run(ps, cs, R, temps, xyz, steps, exchange_every=exchange, mala_step=h, seed=17, trace=True)
+    gt, ot = g["trace"], o["trace"]
+    assert np.array_equal(gt["bond"], ot["bond"])
+    assert np.max(np.abs(gt["delta"] - ot["delta"])) <= 1e-6
+    assert np.array_equal(gt["accepted"], ot["accepted"]), np.argwhere(gt["accepted"] != ot["accepted"])[:5]
+    for k in range(len(ps) * R):
+        scale = _l1(list(O.energy_terms(ps[k // R], cs[k // R]).values()))
+        # Rejected proposals that run into the r^-12 wall (|dE| above the
+        # start's own terms) are left out: there the 1e-6 rad difference of
+        # the proposals (drift carries the FP32 gradient) moves the energy by
+        # more than its FP32 error.  Accepted steps and every moderate
+        # proposal are compared.
+        sel = (gt["accepted"][k] == 1) | (np.abs(ot["dE"][k]) <= scale)
+        err = np.abs(gt["dE"][k] - ot["dE"][k])[sel]
+        assert np.all(err <= 1e-5 * (scale + np.abs(ot["dE"][k][sel]))), k
+        assert g["stats"][k]["slot"] == o["stats"][k]["slot"]
+        assert g["stats"][k]["accepted"] == o["stats"][k]["accepted"]
+        assert g["stats"][k]["status"] == 0
+        assert abs(g["stats"][k]["e"] - o["stats"][k]["e"]) <= 1e-5 * scale
+        assert np.max(np.abs(g["xyz"][k] - o["xyz"][k])) <= 1e-5
+    assert np.array_equal(g["swaps"], o["swaps"])
+    if exchange:
+        assert g["swaps"][:, 0].sum() > 0
+    # a second call continues the step counters (Philox counters) and states
+    g2 = ctx.pt_run(5, exchange_every=exchange, mala_step=h, seed=17)
+    assert all(st["moves"] >= 30 for st in g2["stats"])
+
+
+def test_mc_run_mala_mode_is_the_same_sampler(ctx):
+    from alma import _native as N
+    s = S.chain(24, 3)
+    c = S.csr(S.topology(s))
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    r = ctx.mc_run(N.MC_MALA, 25, n_chains=2, temperature=KT, seed=5, mala_step=0.05, trace=True)
+    o = O.pt_run([s], [c], 2, [KT, KT], [s["xyz"], s["xyz"]], 25, mala_step=0.05, seed=5, trace=True)
+    assert np.array_equal(r["trace"]["accepted"], o["trace"]["accepted"][0])
+    assert np.array_equal(r["trace"]["bond"], o["trace"]["bond"][0])
+    for k in range(2):
+        assert np.max(np.abs(r["xyz"][k] - o["xyz"][k])) <= 1e-5
+        assert r["stats"][k]["accepted"] == o["stats"][k]["accepted"]
+
+
+def test_pt_rejects_bad_arguments(ctx):
+    ps, cs = _proteins()
+    ctx.pt_set(ps[:1], cs[:1], 1, [KT], [ps[0]["xyz"]])
+    with pytest.raises(ValueError):
+        ctx.pt_run(3, mala_step=0.0)
+    with pytest.raises(ValueError):
+        ctx.pt_set(ps[:1], cs[:1], 1, [-1.0], [ps[0]["xyz"]])
