@@ -241,9 +241,9 @@ void upload_positions(alma_ctx* ctx) {
     HIP_TRY(hipMemcpy(s.pos, p4.data(), sizeof(double) * p4.size(), hipMemcpyHostToDevice));
 }
 
-// neighbour list + Born radii + terms, growing the per-atom capacity when a
-// row overflows
-void evaluate(alma_ctx* ctx, double* terms) {
+// neighbour list + Born radii, growing the per-atom capacity when a row
+// overflows
+void prepare(alma_ctx* ctx) {
     DevSys& s = ctx->sys;
     for (int attempt = 0; attempt < 3; ++attempt) {
         HIP_TRY(launch_build_nl(s, ctx->stream));
@@ -257,15 +257,23 @@ void evaluate(alma_ctx* ctx, double* terms) {
             continue;
         }
         HIP_TRY(launch_born(s, ctx->stream));
+        return;
+    }
+    throw Fail{ALMA_ERUNTIME, "neighbour list capacity could not be sized"};
+}
+
+// neighbour list + Born radii + terms
+void evaluate(alma_ctx* ctx, double* terms) {
+    DevSys& s = ctx->sys;
+    {
+        prepare(ctx);
         HIP_TRY(launch_energy(s, ctx->stream));
         if (terms)
             HIP_TRY(hipMemcpyAsync(terms, s.terms, sizeof(double) * ALMA_NTERMS, hipMemcpyDeviceToHost,
                                    ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         ctx->have_eval = true;
-        return;
     }
-    throw Fail{ALMA_ERUNTIME, "neighbour list capacity could not be sized"};
 }
 
 // ─────────────────────────────── MC driver ─────────────────────────────────
@@ -313,6 +321,7 @@ void free_chains(ChainBatch& b) {
     dev_free(b.nlh);
     dev_free(b.nlf);
     dev_free(b.nls);
+    dev_free(b.bm);
     dev_free(b.nlh_cnt);
     dev_free(b.nlf_cnt);
     dev_free(b.nls_cnt);
@@ -342,6 +351,7 @@ void alloc_rows(ChainBatch& b) {
     dev_alloc(&b.nlh, rows * b.cap_h);
     dev_alloc(&b.nlf, rows * b.cap_f);
     dev_alloc(&b.nls, rows * b.cap_s);
+    dev_alloc(&b.bm, rows * (size_t)((b.n + 31) / 32));
 }
 
 void alloc_chains(ChainBatch& b, const DevSys& s, int nch, int trace_steps) {
@@ -395,18 +405,7 @@ int pow2_at_least(int v) {
     return p;
 }
 
-void rebuild_chains(alma_ctx* ctx, const std::vector<int>& ids, int count) {
-    ChainBatch& b = ctx->chains;
-    HIP_TRY(hipMemcpyAsync(b.list, ids.data(), sizeof(int) * ids.size(), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(launch_chain_rebuild(b, ctx->sys, (int)ids.size(), count, ctx->stream));
-    std::vector<int> fl(4 * (size_t)b.nch);
-    HIP_TRY(hipMemcpyAsync(fl.data(), b.flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (int c : ids) {
-        if (fl[4 * c] & 2) throw Fail{ALMA_ERUNTIME, "chain expanded beyond its cell grid"};
-        if (fl[4 * c] & 5) throw Fail{ALMA_ERUNTIME, "neighbour rows exceeded their capacity"};
-    }
-}
+
 
 void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_stats* stats, double* snap_xyz,
             double* snap_e) {
@@ -474,19 +473,19 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     };
     for (int c = 0; c < nch; ++c) st[c].steps_target = target_of(c);
     HIP_TRY(hipMemcpy(b.st, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice));
-    // initial lists (the first build is not a rebuild)
-    std::vector<int> all(nch);
-    for (int c = 0; c < nch; ++c) all[c] = c;
-    rebuild_chains(ctx, all, 0);
+    // the first lists of every chain are built by the MC kernel itself
+    // (status CS_REBUILD at launch; not counted as a rebuild)
 
     std::vector<double> snap_pos(4 * (size_t)n);
     for (int round = 0;; ++round) {
         HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
-        std::vector<int> rb, run;
+        std::vector<int> run;
         bool dirty = false;
         for (int c = 0; c < nch; ++c) {
             ChainState& x = st[c];
             if (x.status == CS_ERROR) throw Fail{ALMA_ERUNTIME, "MC chain ran out of random words"};
+            if (x.status == CS_GRID) throw Fail{ALMA_ERUNTIME, "chain expanded beyond its cell grid"};
+            if (x.status == CS_CAPACITY) throw Fail{ALMA_ERUNTIME, "neighbour rows exceeded their capacity"};
             if (x.status == CS_WORDS) {
                 // keep the unconsumed tail, append fresh words
                 int tail = x.wavail - x.cursor;
@@ -527,21 +526,10 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
                 x.status = CS_RUN;
                 dirty = true;
             }
-            if (x.status == CS_REBUILD) rb.push_back(c);
             if (x.status == CS_RUN || x.status == CS_REBUILD) run.push_back(c);
         }
         if (run.empty()) break;
         if (dirty) HIP_TRY(hipMemcpy(b.st, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice));
-        if (!rb.empty()) {
-            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-            rebuild_chains(ctx, rb, 1);
-            HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-            HIP_TRY(hipEventSynchronize(ctx->ev1));
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            ctx->t_rebuild_ms += ms;
-            ctx->n_rebuild_launch += 1;
-        }
         HIP_TRY(hipMemcpyAsync(b.list, run.data(), sizeof(int) * run.size(), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
         HIP_TRY(launch_chain_mc(b, s, (int)run.size(), ctx->stream));
@@ -554,7 +542,16 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
             ctx->n_mc_launch += 1;
         }
     }
+    HIP_TRY(launch_chain_energy(b, s, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
     for (int k = 0; k < 5; ++k) ctx->work[k] = 0;
+    // list rebuilds run inside the MC kernel: chain-time spent in them
+    // (wall clock of each chain's workgroup) and their count
+    for (int c = 0; c < nch; ++c) {
+        ctx->t_rebuild_ms += st[c].clk[7] / 1e5;
+        ctx->n_rebuild_launch += st[c].rebuilds;
+    }
     for (int c = 0; c < nch; ++c) {
         ctx->work[0] += st[c].n_cross;
         ctx->work[1] += st[c].n_pair_eval;
@@ -568,12 +565,13 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
         for (int k = 0; k < 7; ++k) tot += x.clk[k];
         std::fprintf(stderr,
                      "[alma mc] chain0 steps=%d torsion=%lld crank=%lld conc=%lld  us/step: select %.2f move %.2f "
-                     "segments %.2f passA %.2f radii %.2f passB %.2f commit %.2f total %.2f\n",
+                     "segments %.2f passA %.2f radii %.2f passB %.2f commit %.2f total %.2f  rebuild %.2f us x %lld\n",
                      x.steps_done, x.steps_by_kind[0], x.steps_by_kind[1], x.steps_by_kind[2],
                      x.clk[0] / 100.0 / x.steps_done, x.clk[1] / 100.0 / x.steps_done,
                      x.clk[2] / 100.0 / x.steps_done, x.clk[3] / 100.0 / x.steps_done,
                      x.clk[4] / 100.0 / x.steps_done, x.clk[5] / 100.0 / x.steps_done,
-                     x.clk[6] / 100.0 / x.steps_done, tot / 100.0 / x.steps_done);
+                     x.clk[6] / 100.0 / x.steps_done, tot / 100.0 / x.steps_done,
+                     x.rebuilds ? x.clk[7] / 100.0 / x.rebuilds : 0.0, x.rebuilds);
     }
     // results
     std::vector<double> p4(4 * (size_t)n);
@@ -1315,9 +1313,7 @@ alma_status alma_forces(alma_ctx* ctx, double* energy, double* dEdx, double* dEd
             *energy = 0.0;
             return;
         }
-        double terms[ALMA_NTERMS];
-        evaluate(ctx, terms);
-        *energy = terms[ALMA_NTERMS - 1];
+        prepare(ctx);
         if (ctx->f_n != s.n) {
             free_forces(ctx);
             dev_alloc(&ctx->f_grad, 3 * (size_t)s.n);
@@ -1329,6 +1325,9 @@ alma_status alma_forces(alma_ctx* ctx, double* energy, double* dEdx, double* dEd
         const bool want_t = dEdtheta && s.has_topo && s.topo.n_rot > 0;
         HIP_TRY(launch_forces(s, ctx->f_grad, ctx->f_dEda, ctx->f_pre, want_t ? ctx->f_dEdt : nullptr,
                               ctx->stream));
+        HIP_TRY(hipMemcpyAsync(energy, s.terms + (ALMA_NTERMS - 1), sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        ctx->have_eval = true;
         HIP_TRY(hipMemcpyAsync(dEdx, ctx->f_grad, sizeof(double) * 3 * s.n, hipMemcpyDeviceToHost, ctx->stream));
         if (want_t)
             HIP_TRY(hipMemcpyAsync(dEdtheta, ctx->f_dEdt, sizeof(double) * s.topo.n_rot, hipMemcpyDeviceToHost,

@@ -98,11 +98,12 @@ struct DevSys {
 
 // ---- device MC (mc.hip) ----
 // Chain status codes written by the MC kernel.
-enum { CS_RUN = 0, CS_REBUILD = 1, CS_TARGET = 2, CS_WORDS = 3, CS_ERROR = 4 };
+enum { CS_RUN = 0, CS_REBUILD = 1, CS_TARGET = 2, CS_WORDS = 3, CS_ERROR = 4, CS_GRID = 5, CS_CAPACITY = 6 };
 
 // State of one MC chain (device resident).
 struct ChainState {
-    double cur_e;        // running energy (curE)
+    double cur_e;        // running energy (curE): fresh total at the last list build + de_since
+    double de_since;     // sum of accepted dE since the last list build
     double cur_max;      // adaptive step
     double temperature;  // kT, kcal/mol
     double mala_step;
@@ -114,7 +115,7 @@ struct ChainState {
     int status;
     int acc_win, tot_win;
     int mode;
-    long long clk[8];    // per-phase clock ticks (select, move, segments, pass A, radii, pass B, commit, total)
+    long long clk[8];    // per-phase clock ticks (select, move, segments, pass A, radii, pass B, commit, list rebuild)
     long long steps_by_kind[3];
     // algorithmic work census: cross pairs (HCT change), in-cutoff pair
     // evaluations (old + new), SASA row entries folded, moved atoms,
@@ -141,6 +142,7 @@ struct ChainBatch {
     uint32_t* nlh = nullptr;  // nch * n * cap_h half rows, cell-scan order, excl bit
     uint32_t* nlf = nullptr;  // nch * n * cap_f full rows, ascending index, excl bit
     uint32_t* nls = nullptr;  // nch * n * cap_s half rows within SASA reach + skin
+    uint32_t* bm = nullptr;   // nch * n * ceil(n/32) neighbour bitmap (list rebuild scratch)
     int* nlh_cnt = nullptr;
     int* nlf_cnt = nullptr;
     int* nls_cnt = nullptr;
@@ -265,15 +267,18 @@ hipError_t launch_batch_step(const Batch& b, int init, hipStream_t st);
 // evaluation kernels only, pair kernel variant chosen by lanes / block
 hipError_t launch_batch_pair(const Batch& b, int lanes, int block, hipStream_t st);
 
-hipError_t launch_chain_rebuild(ChainBatch& b, const DevSys& s, int nlist, int count, hipStream_t st);
 hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int nlist, hipStream_t st);
+// curE of every chain: total energy at its last list build + accepted dE since
+hipError_t launch_chain_energy(ChainBatch& b, const DevSys& s, hipStream_t st);
 
 // ---- kernels / launchers (energy.hip) ----
 hipError_t launch_build_nl(DevSys& s, hipStream_t st);
 hipError_t launch_born(DevSys& s, hipStream_t st);
 hipError_t launch_energy(DevSys& s, hipStream_t st);
-// gradient of the total energy at the positions of the last launch_energy:
-// G (3n), dEda (n scratch), pre (6(n+1) scratch), dEdt (n_rot, may be null)
+hipError_t launch_reduce_terms(DevSys& s, hipStream_t st);
+// energy terms and gradient at the positions of the last list build and
+// launch_born: G (3n), dEda (n scratch), pre (6(n+1) scratch), dEdt (n_rot,
+// may be null); terms land in s.terms as launch_energy leaves them
 hipError_t launch_forces(DevSys& s, double* G, double* dEda, double* pre, double* dEdt, hipStream_t st);
 
 }  // namespace alma

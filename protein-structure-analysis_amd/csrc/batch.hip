@@ -493,9 +493,10 @@ __global__ void kb_pair(Batch b) {
     }
 }
 
-// ── SASA (sasa_nonpolar :2068-2083) and the adjoint of its ordered fold ─────
-__global__ void kb_sasa(Batch b) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
+// ── SASA (sasa_nonpolar :2068-2083) and the adjoint of its ordered fold,
+// one wavefront per atom ─────────────────────────────────────────────────────
+__global__ void __launch_bounds__(kBlock) kb_sasa(Batch b) {
+    int g = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
     if (g >= b.n_atoms) return;
     const BSys& sy = b.sys[b.atom_sys[g]];
     const int i = g - sy.aoff;
@@ -504,57 +505,17 @@ __global__ void kb_sasa(Batch b) {
     double* G = b.Gn + 3 * (size_t)sy.aoff;
     const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
     const double ri = (double)prm[i].y + kProbe;
-    const double k2 = 2.0 * M_PI * ri * ri / (2.0 * ri);
-    const int m = b.nl_cnt[g];
     const uint32_t* row = b.nl + (size_t)g * b.cap;
-    double sa = 4.0 * M_PI * ri * ri;
-    int clamped = 0;
-    for (int k = 0; k < m; ++k) {
-        int j = (int)(row[k] & kIdxMask);
-        double dc = ri + (double)prm[j].y + kProbe;
-        double r2 = dist2_exact(pi, x + 4 * j);
-        if (r2 >= dc * dc) continue;
-        double r = (double)sqrtf((float)r2);
-        if (r >= dc) continue;
-        double cap = k2 * (dc - r);
-        if (sa * 0.85 < cap) {
-            sa -= sa * 0.85;
-            ++clamped;
-        } else {
-            sa -= cap;
-        }
-    }
-    b.part[5 * (size_t)g + 4] = kGammaSA * fmax(0.0, sa);
+    const int m = b.nl_cnt[g];
+    int clamped;
+    double sa = sasa_fold_wave(pi, x, prm, ri, row, m, &clamped);
+    if (lane_id() == 0) b.part[5 * (size_t)g + 4] = kGammaSA * fmax(0.0, sa);
     if (!(sa > 0.0)) return;
-    double g0 = 0, g1 = 0, g2 = 0;
-    sa = 4.0 * M_PI * ri * ri;
-    int before = 0;
-    for (int k = 0; k < m; ++k) {
-        int j = (int)(row[k] & kIdxMask);
-        double dc = ri + (double)prm[j].y + kProbe;
-        double r2 = dist2_exact(pi, x + 4 * j);
-        if (r2 >= dc * dc) continue;
-        double r = (double)sqrtf((float)r2);
-        if (r >= dc) continue;
-        double cap = k2 * (dc - r);
-        if (sa * 0.85 < cap) {
-            sa -= sa * 0.85;
-            ++before;
-            continue;
-        }
-        sa -= cap;
-        double sc = kGammaSA * k2 * pow(0.15, (double)(clamped - before)) / sqrt(r2);
-        double u0 = (pi[0] - x[4 * j]) * sc, u1 = (pi[1] - x[4 * j + 1]) * sc, u2 = (pi[2] - x[4 * j + 2]) * sc;
-        g0 += u0;
-        g1 += u1;
-        g2 += u2;
-        atomicAdd(&G[3 * j], -u0);
-        atomicAdd(&G[3 * j + 1], -u1);
-        atomicAdd(&G[3 * j + 2], -u2);
-    }
-    atomicAdd(&G[3 * i], g0);
-    atomicAdd(&G[3 * i + 1], g1);
-    atomicAdd(&G[3 * i + 2], g2);
+    double gi[3] = {0, 0, 0};
+    sasa_adjoint_wave(pi, x, prm, ri, row, m, clamped, G, gi);
+    for (int d = 0; d < 3; ++d) gi[d] = wave_sum(gi[d]);
+    if (lane_id() == 0)
+        for (int d = 0; d < 3; ++d) atomicAdd(&G[3 * i + d], gi[d]);
 }
 
 // d hct(r; ri, rj) / dr
@@ -839,7 +800,7 @@ hipError_t launch_batch_step(const Batch& b, int init, hipStream_t st) {
     kb_born_fin<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
     hipError_t e = launch_batch_pair(b, b.lanes, b.pair_block, st);
     if (e != hipSuccess) return e;
-    kb_sasa<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
+    kb_sasa<<<blocks_for(N, kWavesPerBlock), kBlock, 0, st>>>(b);
     kb_bornchain<<<blocks_for(N, kWavesPerBlock), kBlock, 0, st>>>(b);
     if (b.n_rec > 0) kb_bonded<<<blocks_for(b.n_rec, kBlock), kBlock, 0, st>>>(b);
     kb_accept<<<b.n_sys, kSysBlock, 0, st>>>(b, init);

@@ -269,28 +269,17 @@ __global__ void __launch_bounds__(kBlock) k_pair_energy(const double* __restrict
 // ── SASA ────────────────────────────────────────────────────────────────────
 // sasa_nonpolar, src/physics_engine.cpp:2068-2083: atom i is occluded only by
 // its half-list partners, folded in neighbour-list order.  One lane per atom.
-__global__ void k_sasa(const double* __restrict__ pos, const float4* __restrict__ prm, int n,
-                       const uint32_t* __restrict__ nl, const int* __restrict__ cnt, int cap,
-                       double* __restrict__ part) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per atom: ordered fold, contacts tested 64 at a time
+__global__ void __launch_bounds__(kBlock) k_sasa(const double* __restrict__ pos, const float4* __restrict__ prm,
+                                                int n, const uint32_t* __restrict__ nl,
+                                                const int* __restrict__ cnt, int cap, double* __restrict__ part) {
+    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
     if (i >= n) return;
     const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
     double ri = (double)prm[i].y + kProbe;
-    double sa = 4.0 * M_PI * ri * ri;
-    const double k2 = 2.0 * M_PI * ri * ri / (2.0 * ri);
-    int m = min(cnt[i], cap);
-    const uint32_t* row = nl + (size_t)i * cap;
-    for (int k = 0; k < m; ++k) {
-        int j = (int)(row[k] & kIdxMask);
-        double rj = (double)prm[j].y + kProbe;
-        double dc = ri + rj;
-        double r2 = dist2_exact(pi, pos + 4 * j);
-        if (r2 >= dc * dc) continue;
-        double r = (double)sqrtf((float)r2);
-        if (r >= dc) continue;
-        sa -= fmin(sa * 0.85, k2 * (dc - r));
-    }
-    part[5 * i + 4] = kGammaSA * fmax(0.0, sa);
+    int clamped;
+    double sa = sasa_fold_wave(pi, pos, prm, ri, nl + (size_t)i * cap, min(cnt[i], cap), &clamped);
+    if (lane_id() == 0) part[5 * i + 4] = kGammaSA * fmax(0.0, sa);
 }
 
 __global__ void k_bonded(const double* __restrict__ pos, DevTopo t, double* __restrict__ rec) {
@@ -377,10 +366,14 @@ hipError_t launch_energy(DevSys& s, hipStream_t st) {
     const int n = s.n;
     k_pair_energy<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s.pos, s.prm, s.rstar, s.eps64, n, s.nl, s.nl_cnt,
                                                                     s.cap, s.has_topo ? 1 : 0, s.part);
-    k_sasa<<<blocks_for(n, kBlock), kBlock, 0, st>>>(s.pos, s.prm, n, s.nl, s.nl_cnt, s.cap, s.part);
+    k_sasa<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s.pos, s.prm, n, s.nl, s.nl_cnt, s.cap, s.part);
     int nrec = s.has_topo ? s.topo.n_dih + s.topo.n_ss : 0;
     if (nrec > 0) k_bonded<<<blocks_for(nrec, kBlock), kBlock, 0, st>>>(s.pos, s.topo, s.rec);
-    k_reduce_terms<<<1, kBlock, 0, st>>>(n, s.part, s.has_topo ? s.topo.n_dih : 0,
+    return launch_reduce_terms(s, st);
+}
+
+hipError_t launch_reduce_terms(DevSys& s, hipStream_t st) {
+    k_reduce_terms<<<1, kBlock, 0, st>>>(s.n, s.part, s.has_topo ? s.topo.n_dih : 0,
                                          s.has_topo ? s.topo.n_ss : 0, s.rec, s.terms);
     return hipGetLastError();
 }

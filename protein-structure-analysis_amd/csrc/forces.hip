@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(kBlock) k_f_pair(DevSys s, double* __restrict_
     float4 qi = s.prm[i];
     int m = min(s.nl_cnt[i], s.cap);
     const uint32_t* row = s.nl + (size_t)i * s.cap;
-    double gi[3] = {0, 0, 0}, dai = 0.0;
+    double gi[3] = {0, 0, 0}, dai = 0.0, dh = 0, gb = 0, lj = 0, hc = 0;
     for (int k = lane; k < m; k += kWave) {
         uint32_t e = row[k];
         if (s.has_topo && (e & kExclBit)) continue;
@@ -91,6 +91,11 @@ __global__ void __launch_bounds__(kBlock) k_f_pair(DevSys s, double* __restrict_
         double r2 = dist2_exact(pi, pos + 4 * j);
         if (r2 > kPairCut2) continue;
         float4 qj = s.prm[j];
+        PairTerms pt = pair_terms(r2, qi, qj, s.rstar[i], s.rstar[j], s.eps64[i], s.eps64[j]);
+        dh += pt.dh;
+        gb += pt.gb;
+        lj += pt.lj;
+        hc += pt.hc;
         double dr, dD;
         dpair(r2, qi, qj, s.rstar[i], s.rstar[j], s.eps64[i], s.eps64[j], &dr, &dD);
         double r = sqrt(r2);
@@ -102,9 +107,18 @@ __global__ void __launch_bounds__(kBlock) k_f_pair(DevSys s, double* __restrict_
     }
     for (int d = 0; d < 3; ++d) gi[d] = wave_sum(gi[d]);
     dai = wave_sum(dai);
+    dh = wave_sum(dh);
+    gb = wave_sum(gb);
+    lj = wave_sum(lj);
+    hc = wave_sum(hc);
     if (lane == 0) {
         for (int d = 0; d < 3; ++d) atomicAdd(&G[3 * i + d], gi[d]);
         atomicAdd(&dEda[i], dai);
+        double* part = s.part + 5 * (size_t)i;
+        part[0] = dh;
+        part[1] = gb;
+        part[2] = lj;
+        part[3] = hc;
     }
 }
 
@@ -142,63 +156,23 @@ __global__ void __launch_bounds__(kBlock) k_f_born(DevSys s, const double* __res
         for (int d = 0; d < 3; ++d) atomicAdd(&G[3 * i + d], gi[d]);
 }
 
-// pass 3: SASA fold adjoint, one lane per atom.  The fold
-//   sa <- sa - min(0.85 sa, pi R_i (dc - r))
-// is replayed twice with the same arithmetic as the energy kernel: the first
-// pass counts the clamped steps, the second gives every unclamped contact
-// dE/dr = gamma pi R_i 0.15^(clamped steps after it).
-__global__ void k_f_sasa(DevSys s, double* __restrict__ G) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// pass 3: SASA fold adjoint, one wavefront per atom
+__global__ void __launch_bounds__(kBlock) k_f_sasa(DevSys s, double* __restrict__ G) {
+    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
     if (i >= s.n) return;
-    const double* pos = s.pos;
-    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
-    double ri = (double)s.prm[i].y + kProbe;
-    const double k2 = 2.0 * M_PI * ri * ri / (2.0 * ri);
-    int m = min(s.nl_cnt[i], s.cap);
+    const double pi[3] = {s.pos[4 * i], s.pos[4 * i + 1], s.pos[4 * i + 2]};
+    const double ri = (double)s.prm[i].y + kProbe;
     const uint32_t* row = s.nl + (size_t)i * s.cap;
-    double sa = 4.0 * M_PI * ri * ri;
-    int clamped = 0;
-    for (int k = 0; k < m; ++k) {
-        int j = (int)(row[k] & kIdxMask);
-        double dc = ri + (double)s.prm[j].y + kProbe;
-        double r2 = dist2_exact(pi, pos + 4 * j);
-        if (r2 >= dc * dc) continue;
-        double r = (double)sqrtf((float)r2);
-        if (r >= dc) continue;
-        double cap = k2 * (dc - r);
-        if (sa * 0.85 < cap) {
-            sa -= sa * 0.85;
-            ++clamped;
-        } else {
-            sa -= cap;
-        }
-    }
+    const int m = min(s.nl_cnt[i], s.cap);
+    int clamped;
+    double sa = sasa_fold_wave(pi, s.pos, s.prm, ri, row, m, &clamped);
+    if (lane_id() == 0) s.part[5 * (size_t)i + 4] = kGammaSA * fmax(0.0, sa);
     if (!(sa > 0.0)) return;
     double gi[3] = {0, 0, 0};
-    sa = 4.0 * M_PI * ri * ri;
-    int before = 0;
-    for (int k = 0; k < m; ++k) {
-        int j = (int)(row[k] & kIdxMask);
-        double dc = ri + (double)s.prm[j].y + kProbe;
-        double r2 = dist2_exact(pi, pos + 4 * j);
-        if (r2 >= dc * dc) continue;
-        double r = (double)sqrtf((float)r2);
-        if (r >= dc) continue;
-        double cap = k2 * (dc - r);
-        if (sa * 0.85 < cap) {
-            sa -= sa * 0.85;
-            ++before;
-            continue;
-        }
-        sa -= cap;
-        double sc = kGammaSA * k2 * pow(0.15, (double)(clamped - before)) / sqrt(r2);
-        double u[3] = {(pi[0] - pos[4 * j]) * sc, (pi[1] - pos[4 * j + 1]) * sc, (pi[2] - pos[4 * j + 2]) * sc};
-        for (int d = 0; d < 3; ++d) gi[d] += u[d];
-        atomicAdd(&G[3 * j], -u[0]);
-        atomicAdd(&G[3 * j + 1], -u[1]);
-        atomicAdd(&G[3 * j + 2], -u[2]);
-    }
-    for (int d = 0; d < 3; ++d) atomicAdd(&G[3 * i + d], gi[d]);
+    sasa_adjoint_wave(pi, s.pos, s.prm, ri, row, m, clamped, G, gi);
+    for (int d = 0; d < 3; ++d) gi[d] = wave_sum(gi[d]);
+    if (lane_id() == 0)
+        for (int d = 0; d < 3; ++d) atomicAdd(&G[3 * i + d], gi[d]);
 }
 
 __device__ __forceinline__ void cross3(const double* u, const double* v, double* w) {
@@ -227,11 +201,15 @@ __global__ void k_f_bonded(DevSys s, double* __restrict__ G) {
         double X = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
         double Y = m1[0] * n2[0] + m1[1] * n2[1] + m1[2] * n2[2];
         double den = X * X + Y * Y;
+        double phi = dih_angle_d(pos, a, b, c, d);
+        double gphi = 0.0, e = 0.0;
+        for (int m = t.dih_toff[k]; m < t.dih_toff[k + 1]; ++m) {
+            double arg = (double)t.dih_n[m] * phi - t.dih_gamma[m];
+            e += t.dih_v2[m] * (1.0 + cos(arg));
+            gphi += -t.dih_v2[m] * t.dih_n[m] * sin(arg);
+        }
+        s.rec[k] = e;
         if (den == 0.0) return;
-        double phi = atan2(Y, X);
-        double gphi = 0.0;
-        for (int m = t.dih_toff[k]; m < t.dih_toff[k + 1]; ++m)
-            gphi += -t.dih_v2[m] * t.dih_n[m] * sin((double)t.dih_n[m] * phi - t.dih_gamma[m]);
         double gX = -Y / den * gphi, gY = X / den * gphi;
         double gn1[3], gn2[3], gm1[3], gb1[3] = {0, 0, 0}, gb2[3] = {0, 0, 0}, gb3[3] = {0, 0, 0}, tt[3];
         for (int q = 0; q < 3; ++q) {
@@ -260,7 +238,9 @@ __global__ void k_f_bonded(DevSys s, double* __restrict__ G) {
     } else if (k < t.n_dih + t.n_ss) {
         int q0 = k - t.n_dih;
         int i = t.ss_i[q0], j = t.ss_j[q0];
-        double r = sqrt(dist2_exact(pos + 4 * i, pos + 4 * j));
+        double dx = pos[4 * i] - pos[4 * j], dy = pos[4 * i + 1] - pos[4 * j + 1], dz = pos[4 * i + 2] - pos[4 * j + 2];
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        s.rec[k] = kKss * (r - kR0ss) * (r - kR0ss);
         double sc = 2.0 * kKss * (r - kR0ss) / r;
         for (int q = 0; q < 3; ++q) {
             double v = sc * (pos[4 * i + q] - pos[4 * j + q]);
@@ -270,42 +250,45 @@ __global__ void k_f_bonded(DevSys s, double* __restrict__ G) {
     }
 }
 
-// pass 5: inclusive prefix sums over atoms of x_i x G_i and G_i (6 lanes of
-// FP64), single block, then dE/dtheta_k from each side's runs
-__global__ void k_f_prefix(DevSys s, const double* __restrict__ G, double* __restrict__ pre) {
-    __shared__ double buf[1024];
+// pass 5: inclusive prefix sums over atoms of x_i x G_i and G_i, one block
+// per component (6 lanes of FP64): wavefront scans, then a scan of the 16
+// wavefront totals; then dE/dtheta_k from each side's runs
+__global__ void __launch_bounds__(1024) k_f_prefix(DevSys s, const double* __restrict__ G,
+                                                   double* __restrict__ pre) {
+    __shared__ double wtot[16];
+    const int comp = blockIdx.x;
     const int n = s.n;
     const double* pos = s.pos;
-    for (int comp = 0; comp < 6; ++comp) {
-        double carry = 0.0;
-        for (int base = 0; base < n; base += 1024) {
-            int i = base + threadIdx.x;
-            double v = 0.0;
-            if (i < n) {
-                if (comp < 3) {
-                    double x[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]}, g[3] = {G[3 * i], G[3 * i + 1],
-                                                                                         G[3 * i + 2]};
-                    int a1 = (comp + 1) % 3, a2 = (comp + 2) % 3;
-                    v = x[a1] * g[a2] - x[a2] * g[a1];
-                } else {
-                    v = G[3 * i + comp - 3];
-                }
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    double* out = pre + (size_t)comp * (n + 1);
+    double carry = 0.0;
+    for (int base = 0; base < n; base += 1024) {
+        int i = base + threadIdx.x;
+        double v = 0.0;
+        if (i < n) {
+            if (comp < 3) {
+                int a1 = (comp + 1) % 3, a2 = (comp + 2) % 3;
+                v = pos[4 * i + a1] * G[3 * i + a2] - pos[4 * i + a2] * G[3 * i + a1];
+            } else {
+                v = G[3 * i + comp - 3];
             }
-            buf[threadIdx.x] = v;
-            __syncthreads();
-            for (int o = 1; o < 1024; o <<= 1) {
-                double add = threadIdx.x >= (unsigned)o ? buf[threadIdx.x - o] : 0.0;
-                __syncthreads();
-                buf[threadIdx.x] += add;
-                __syncthreads();
-            }
-            if (i < n) pre[(size_t)comp * (n + 1) + i + 1] = carry + buf[threadIdx.x];
-            double tot = buf[1023];
-            __syncthreads();
-            carry += tot;
         }
-        if (threadIdx.x == 0) pre[(size_t)comp * (n + 1)] = 0.0;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            double t = __shfl_up(v, o, kWave);
+            if (lane >= o) v += t;
+        }
+        if (lane == kWave - 1) wtot[w] = v;
+        __syncthreads();
+        double before = carry;
+        for (int k = 0; k < w; ++k) before += wtot[k];
+        if (i < n) out[i + 1] = before + v;
+        double tot = 0.0;
+        for (int k = 0; k < 16; ++k) tot += wtot[k];
+        __syncthreads();
+        carry += tot;
     }
+    if (threadIdx.x == 0) out[0] = 0.0;
 }
 
 __global__ void k_f_torsion(DevSys s, const double* __restrict__ pre, double* __restrict__ dEdt) {
@@ -350,19 +333,24 @@ inline unsigned blocks_for(int n, int per) { return (unsigned)((n + per - 1) / p
 
 }  // namespace
 
+// Energy terms and gradient in one sequence after the list and Born radii:
+// the pair, SASA and bonded passes write the per-atom / per-record energy
+// partials the energy path would, then the terms are reduced.
 hipError_t launch_forces(DevSys& s, double* G, double* dEda, double* pre, double* dEdt, hipStream_t st) {
     const int n = s.n;
     k_f_zero<<<blocks_for(4 * n, kBlock), kBlock, 0, st>>>(G, 3 * n, dEda, n);
     k_f_pair<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s, G, dEda);
     k_f_born<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s, dEda, G);
-    k_f_sasa<<<blocks_for(n, kBlock), kBlock, 0, st>>>(s, G);
+    k_f_sasa<<<blocks_for(n, kWavesPerBlock), kBlock, 0, st>>>(s, G);
     int nrec = s.has_topo ? s.topo.n_dih + s.topo.n_ss : 0;
     if (nrec > 0) k_f_bonded<<<blocks_for(nrec, kBlock), kBlock, 0, st>>>(s, G);
     if (s.has_topo && s.topo.n_rot > 0 && dEdt) {
-        k_f_prefix<<<1, 1024, 0, st>>>(s, G, pre);
+        k_f_prefix<<<6, 1024, 0, st>>>(s, G, pre);
         k_f_torsion<<<blocks_for(s.topo.n_rot, kBlock), kBlock, 0, st>>>(s, pre, dEdt);
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_reduce_terms(s, st);
 }
 
 }  // namespace alma

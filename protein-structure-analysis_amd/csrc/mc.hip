@@ -41,388 +41,323 @@ __device__ __forceinline__ double* P4(const ChainBatch& b, double* base, int c) 
 }
 __device__ __forceinline__ size_t R1(const ChainBatch& b, int c) { return (size_t)c * b.n; }
 
-// ───────────────────────── batched list rebuild ─────────────────────────────
-
-__global__ void rb_clear(ChainBatch b) {
-    int c = b.list[blockIdx.y];
-    size_t o = R1(b, c);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < b.n; i += gridDim.x * blockDim.x) {
-        b.nlf_cnt[o + i] = 0;
-        b.born_sum[o + i] = 0.0;
-    }
-    size_t cb = (size_t)c * (b.max_cells + 1);
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= b.max_cells; k += gridDim.x * blockDim.x)
-        b.cell_cnt[cb + k] = 0;
-    if (blockIdx.x == 0 && threadIdx.x < 4) b.flags[4 * c + threadIdx.x] = 0;
-}
-
-__global__ void rb_bbox(ChainBatch b, double cs) {
-    int c = b.list[blockIdx.y];
-    const double* pos = b.pos + (size_t)c * 4 * b.n;
-    Grid* g = b.grid + c;
-    __shared__ double slo[3][kBlock], shi[3][kBlock];
-    int t = threadIdx.x;
-    double lo[3], hi[3];
-    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = pos[d];
-    for (int i = t; i < b.n; i += blockDim.x)
-        for (int d = 0; d < 3; ++d) {
-            double v = pos[4 * i + d];
-            lo[d] = fmin(lo[d], v);
-            hi[d] = fmax(hi[d], v);
-        }
-    for (int d = 0; d < 3; ++d) {
-        slo[d][t] = lo[d];
-        shi[d][t] = hi[d];
-    }
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (t < s)
-            for (int d = 0; d < 3; ++d) {
-                slo[d][t] = fmin(slo[d][t], slo[d][t + s]);
-                shi[d][t] = fmax(shi[d][t], shi[d][t + s]);
-            }
-        __syncthreads();
-    }
-    if (t == 0) {
-#pragma clang fp contract(off)
-        int ncell = 1;
-        g->cs = cs;
-        for (int d = 0; d < 3; ++d) {
-            double l = slo[d][0] - cs;
-            double len = shi[d][0] - l + 2.0 * cs;
-            int k = (int)ceil(len / cs);
-            g->lo[d] = l;
-            g->nc[d] = k < 1 ? 1 : k;
-            ncell *= g->nc[d];
-        }
-        g->ncell = ncell;
-    }
-}
-
-__global__ void rb_cell_assign(ChainBatch b) {
-    int c = b.list[blockIdx.y];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n) return;
-    const double* pos = b.pos + (size_t)c * 4 * b.n;
-    const Grid* g = b.grid + c;
-    int cc[3];
-    for (int d = 0; d < 3; ++d) {
-        int v = (int)floor((pos[4 * i + d] - g->lo[d]) / g->cs);
-        cc[d] = min(g->nc[d] - 1, max(0, v));
-    }
-    int id = cc[0] + g->nc[0] * (cc[1] + g->nc[1] * cc[2]);
-    if (g->ncell > b.max_cells) {
-        atomicOr(&b.flags[4 * c], 2);
-        id = 0;
-    }
-    b.cell_of[R1(b, c) + i] = id;
-    atomicAdd(&b.cell_cnt[(size_t)c * (b.max_cells + 1) + id], 1);
-}
-
-__global__ void rb_cell_scan(ChainBatch b) {
-    int c = b.list[blockIdx.y];
-    int* cnt = b.cell_cnt + (size_t)c * (b.max_cells + 1);
-    int* start = b.cell_start + (size_t)c * (b.max_cells + 1);
-    __shared__ int buf[1024];
-    int ncell = min(b.grid[c].ncell, b.max_cells);
-    int t = threadIdx.x;
-    int carry = 0;
-    for (int base = 0; base < ncell; base += 1024) {
-        int v = (base + t < ncell) ? cnt[base + t] : 0;
-        buf[t] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            int add = t >= o ? buf[t - o] : 0;
-            __syncthreads();
-            buf[t] += add;
-            __syncthreads();
-        }
-        if (base + t < ncell) start[base + t] = carry + buf[t] - v;
-        int tot = buf[1023];
-        __syncthreads();
-        carry += tot;
-    }
-    if (t == 0) start[ncell] = carry;
-    for (int k = t; k < ncell; k += 1024) cnt[k] = 0;
-}
-
-__global__ void rb_cell_place(ChainBatch b) {
-    int c = b.list[blockIdx.y];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n) return;
-    int cl = b.cell_of[R1(b, c) + i];
-    size_t cb = (size_t)c * (b.max_cells + 1);
-    b.cell_tmp[R1(b, c) + b.cell_start[cb + cl] + atomicAdd(&b.cell_cnt[cb + cl], 1)] = i;
-}
-
-__global__ void rb_cell_rank(ChainBatch b) {
-    int c = b.list[blockIdx.y];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n) return;
-    int cl = b.cell_of[R1(b, c) + i];
-    size_t cb = (size_t)c * (b.max_cells + 1);
-    int s0 = b.cell_start[cb + cl], s1 = b.cell_start[cb + cl + 1];
-    const int* tmp = b.cell_tmp + R1(b, c);
-    int r = 0;
-    for (int k = s0; k < s1; ++k) r += tmp[k] < i;
-    b.cell_atoms[R1(b, c) + s0 + r] = i;
-}
-
-// Half rows in the reference cell-scan order (NeighborList::build), the SASA
-// sub-row of entries closer than R*_i + R*_j + 2*probe + kSasaSkin at build
-// time, and both directions of every pair appended to the full rows (sorted
-// later).  Accepted coordinates stay within 1 A of the build positions; a
-// trial move may carry atoms further, so the MC step folds over the sub-row
-// only while every moved atom is within kSasaTrial of its build position
-// (then no pair can close by more than kSasaSkin) and over the whole half row
-// otherwise.
-__global__ void __launch_bounds__(kBlock) rb_nl_fill(ChainBatch b, DevSys s) {
-    int c = b.list[blockIdx.y];
-    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
-    int* fl = b.flags + 4 * c;
-    if (i >= b.n || (fl[0] & 2)) return;
-    int lane = lane_id();
-    const double* pos = b.pos + (size_t)c * 4 * b.n;
-    const Grid* g = b.grid + c;
-    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
-    const double ri = s.rstar[i];
-    int cl = b.cell_of[R1(b, c) + i];
-    int cx = cl % g->nc[0], cy = (cl / g->nc[0]) % g->nc[1], cz = cl / (g->nc[0] * g->nc[1]);
-    const int* start = b.cell_start + (size_t)c * (b.max_cells + 1);
-    const int* atoms = b.cell_atoms + R1(b, c);
-    int eb = 0, ee = 0;
-    const int* exo = s.has_topo ? s.topo.excl_off : nullptr;
-    if (exo) {
-        eb = exo[i];
-        ee = exo[i + 1];
-    }
-    uint32_t* rowh = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
-    uint32_t* rows = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
-    int* cntf = b.nlf_cnt + R1(b, c);
-    uint32_t* nlf = b.nlf + R1(b, c) * (size_t)b.cap_f;
-    int nh = 0, ns = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        int z = cz + dz;
-        if (z < 0 || z >= g->nc[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            int y = cy + dy;
-            if (y < 0 || y >= g->nc[1]) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                int x = cx + dx;
-                if (x < 0 || x >= g->nc[0]) continue;
-                int cc = x + g->nc[0] * (y + g->nc[1] * z);
-                int e = start[cc + 1];
-                for (int base = start[cc]; base < e; base += kWave) {
-                    int k = base + lane;
-                    bool keep = false, near = false;
-                    int j = 0;
-                    double d2 = 0.0;
-                    if (k < e) {
-                        j = atoms[k];
-                        if (j > i) {
-                            d2 = dist2_exact(pi, pos + 4 * j);
-                            keep = d2 < kNlRcut2;
-                            double reach = ri + s.rstar[j] + 2.0 * kProbe + kSasaSkin;
-                            near = keep && d2 < reach * reach;
-                        }
-                    }
-                    uint64_t m = __ballot(keep);
-                    uint64_t mn = __ballot(near);
-                    uint64_t below = (1ull << lane) - 1ull;
-                    if (keep) {
-                        uint32_t v = (uint32_t)j;
-                        if (exo && in_sorted(s.topo.excl_idx, eb, ee, j)) v |= kExclBit;
-                        int slot = nh + __popcll(m & below);
-                        if (slot < b.cap_h) rowh[slot] = v;
-                        if (near) {
-                            int ss = ns + __popcll(mn & below);
-                            if (ss < b.cap_s) rows[ss] = v;
-                        }
-                        int fj = atomicAdd(&cntf[j], 1);
-                        if (fj < b.cap_f) nlf[(size_t)j * b.cap_f + fj] = (uint32_t)i | (v & kExclBit);
-                        int fi = atomicAdd(&cntf[i], 1);
-                        if (fi < b.cap_f) nlf[(size_t)i * b.cap_f + fi] = v;
-                    }
-                    nh += __popcll(m);
-                    ns += __popcll(mn);
-                }
-            }
-        }
-    }
-    if (lane == 0) {
-        b.nlh_cnt[R1(b, c) + i] = nh;
-        b.nls_cnt[R1(b, c) + i] = ns;
-        atomicMax(&fl[1], nh);
-        atomicMax(&fl[3], ns);
-        if (nh > b.cap_h || ns > b.cap_s) atomicOr(&fl[0], 1);
-    }
-}
-
-// ascending sort of one full row (bitonic, one wavefront per row)
-__global__ void __launch_bounds__(kSortBlock) rb_full_sort(ChainBatch b) {
-    extern __shared__ uint32_t sh[];
-    int c = b.list[blockIdx.y];
-    int i = blockIdx.x;
-    int cnt = b.nlf_cnt[R1(b, c) + i];
-    int* fl = b.flags + 4 * c;
-    if (threadIdx.x == 0) {
-        atomicMax(&fl[2], cnt);
-        if (cnt > b.cap_f) atomicOr(&fl[0], 4);
-    }
-    cnt = min(cnt, b.cap_f);
-    uint32_t* row = b.nlf + (R1(b, c) + i) * (size_t)b.cap_f;
-    int P = 1;
-    while (P < cnt) P <<= 1;
-    for (int k = threadIdx.x; k < P; k += kSortBlock) sh[k] = k < cnt ? row[k] : 0xffffffffu;
-    __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = threadIdx.x; t < P; t += kSortBlock) {
-                int u = t ^ j;
-                if (u > t) {
-                    uint32_t a = sh[t], bb = sh[u];
-                    uint32_t ka = a & kIdxMask, kb = bb & kIdxMask;
-                    if (a == 0xffffffffu) ka = 0xffffffffu;
-                    if (bb == 0xffffffffu) kb = 0xffffffffu;
-                    bool up = (t & k) == 0;
-                    if ((ka > kb) == up) {
-                        sh[t] = bb;
-                        sh[u] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (int k = threadIdx.x; k < cnt; k += kSortBlock) row[k] = sh[k];
-}
-
-__global__ void __launch_bounds__(kBlock) rb_born_sum(ChainBatch b, DevSys s) {
-    int c = b.list[blockIdx.y];
-    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
-    if (i >= b.n) return;
-    int lane = lane_id();
-    const double* pos = b.pos + (size_t)c * 4 * b.n;
-    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
-    float ri = s.prm[i].y;
-    int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
-    const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
-    double* sum = b.born_sum + R1(b, c);
-    double acc = 0.0;
-    for (int k = lane; k < m; k += kWave) {
-        int j = (int)(row[k] & kIdxMask);
-        float r2 = (float)dist2_exact(pi, pos + 4 * j), r = sqrtf(r2);
-        float rj = s.prm[j].y;
-        acc += (double)hct_f(r, r2, ri, rj);
-        atomicAdd(&sum[j], (double)hct_f(r, r2, rj, ri));
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) {
-        atomicAdd(&sum[i], acc);
-        b.half[R1(b, c) + i] = acc;
-    }
-}
-
-__global__ void rb_born_fin(ChainBatch b, DevSys s) {
-    int c = b.list[blockIdx.y];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n) return;
-    size_t o = R1(b, c) + i;
-    b.a[o] = 1.0 / fmax(1.0 / s.rstar[i] - 0.5 * b.born_sum[o], 2.0);
-    for (int d = 0; d < 4; ++d) b.ref[4 * o + d] = b.pos[4 * o + d];
-    b.dhalf[o] = 0.0;
-    b.flag[o] = 0;
-}
-
 __device__ __forceinline__ float4 atom4(const DevSys& s, const double* a, int i) {
     float4 v = s.prm[i];
     v.w = (float)a[i];
     return v;
 }
 
-__global__ void __launch_bounds__(kBlock) rb_pair(ChainBatch b, DevSys s) {
-    int c = b.list[blockIdx.y];
-    int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
-    if (i >= b.n) return;
-    int lane = lane_id();
-    const double* pos = b.pos + (size_t)c * 4 * b.n;
-    const double* a = b.a + R1(b, c);
-    const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
-    float4 qi = atom4(s, a, i);
-    int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
-    const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
-    double acc = 0.0;
-    for (int k = lane; k < m; k += kWave) {
-        uint32_t e = row[k];
-        if (e & kExclBit) continue;
-        int j = (int)(e & kIdxMask);
-        double r2 = dist2_exact(pi, pos + 4 * j);
-        if (r2 > kPairCut2) continue;
-        acc += pair_energy(r2, qi, atom4(s, a, j), s.rstar[i], s.rstar[j], s.eps64[i], s.eps64[j]);
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) b.part[5 * (R1(b, c) + i)] = acc;
-}
+// ─────────────────── list rebuild inside the chain's workgroup ───────────────
+// NeighborList::build (src/physics_engine.cpp:293-342) for one chain by its
+// own 1024-thread workgroup, followed by a fresh Born / energy evaluation
+// (born_radii :2024-2039, total_e :2205-2234), so a chain never waits for
+// the others: no launch boundary, no host round trip.
+//   half rows  : j > i in the reference cell-scan order, excluded bit set
+//   SASA rows  : the half-row entries closer than R*_i + R*_j + 2 probe +
+//                kSasaSkin (list order kept; the fold is order dependent)
+//   full rows  : both directions, ascending index, from a per-chain
+//                neighbour bitmap (n x ceil(n/32) words, set with atomicOr,
+//                read back row by row with wave prefix counts), so no sort
+// Accepted coordinates stay within 1 A of the build positions; a trial move
+// may carry atoms further, so the MC step folds over the SASA row only while
+// every moved atom is within kSasaTrial of its build position (then no pair
+// can close by more than kSasaSkin) and over the whole half row otherwise.
 
-// SASA fold of atom i over its sub-row; `moved`/`posn` select trial
-// coordinates for atoms of the moved set (used by the MC step as well)
-__device__ double sasa_fold(const ChainBatch& b, const DevSys& s, int c, int i, const double* pi,
-                            const double* pos) {
-    double ri = (double)s.prm[i].y + kProbe;
-    double sa = 4.0 * M_PI * ri * ri;
-    const double k2 = M_PI * ri;
-    int m = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
-    const uint32_t* row = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
-    for (int k = 0; k < m; ++k) {
-        int j = (int)(row[k] & kIdxMask);
-        double dc = ri + (double)s.prm[j].y + kProbe;
-        double r2 = dist2_exact(pi, pos + 4 * j);
-        if (r2 >= dc * dc) continue;
-        double r = (double)sqrtf((float)r2);
-        if (r >= dc) continue;
-        sa -= fmin(sa * 0.85, k2 * (dc - r));
-    }
-    return kGammaSA * fmax(0.0, sa);
-}
-
-__global__ void rb_sasa(ChainBatch b, DevSys s) {
-    int c = b.list[blockIdx.y];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= b.n) return;
-    const double* pos = b.pos + (size_t)c * 4 * b.n;
-    b.part[5 * (R1(b, c) + i) + 4] = sasa_fold(b, s, c, i, pos + 4 * i, pos);
-}
-
-__global__ void rb_total(ChainBatch b, DevSys s, int count_rebuild) {
-    int c = b.list[blockIdx.y];
-    __shared__ double buf[kBlock];
-    int t = threadIdx.x;
-    const double* pos = b.pos + (size_t)c * 4 * b.n;
-    double acc = 0.0;
-    for (int i = t; i < b.n; i += kBlock) acc += b.part[5 * (R1(b, c) + i)] + b.part[5 * (R1(b, c) + i) + 4];
-    if (s.has_topo) {
-        for (int k = t; k < s.topo.n_dih; k += kBlock) {
-            double phi = dih_angle_d(pos, s.topo.dih_a[k], s.topo.dih_b[k], s.topo.dih_c[k], s.topo.dih_d[k]);
-            for (int m = s.topo.dih_toff[k]; m < s.topo.dih_toff[k + 1]; ++m)
-                acc += s.topo.dih_v2[m] * (1.0 + cos((double)s.topo.dih_n[m] * phi - s.topo.dih_gamma[m]));
-        }
-        for (int k = t; k < s.topo.n_ss; k += kBlock) {
-            int i = s.topo.ss_i[k], j = s.topo.ss_j[k];
-            double dr = sqrt(dist2_exact(pos + 4 * i, pos + 4 * j)) - kR0ss;
-            acc += kKss * dr * dr;
-        }
-    }
-    buf[t] = acc;
+// data written by atomics is read back after a device-scope fence (it drops
+// the CU's L1 copies) and a workgroup barrier
+__device__ __forceinline__ void wg_sync_global() {
+    __threadfence();
     __syncthreads();
-    for (int h = kBlock / 2; h > 0; h >>= 1) {
-        if (t < h) buf[t] += buf[t + h];
+}
+
+__device__ __forceinline__ void wave_minmax(double* lo, double* hi) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, kWave));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, kWave));
+        }
+}
+
+__device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int count_rebuild, double* dhalf,
+                              int* flag) {
+    const int n = b.n;
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+    const int W = (n + 31) / 32;
+    const double* pos = b.pos + (size_t)c * 4 * n;
+    double* ref = b.ref + (size_t)c * 4 * n;
+    Grid* g = b.grid + c;
+    int* fl = b.flags + 4 * c;
+    int* cnt = b.cell_cnt + (size_t)c * (b.max_cells + 1);
+    int* start = b.cell_start + (size_t)c * (b.max_cells + 1);
+    int* cell_of = b.cell_of + R1(b, c);
+    int* cell_tmp = b.cell_tmp + R1(b, c);
+    int* atoms = b.cell_atoms + R1(b, c);
+    uint32_t* bm = b.bm + R1(b, c) * (size_t)W;
+    double* sum = b.born_sum + R1(b, c);
+    __shared__ double slo[3][kMcWaves], shi[3][kMcWaves];
+    __shared__ int sbuf[kMcBlock];
+
+    // clear
+    for (int i = tid; i < n; i += kMcBlock) sum[i] = 0.0;
+    for (int k = tid; k <= b.max_cells; k += kMcBlock) cnt[k] = 0;
+    for (size_t w = tid; w < (size_t)n * W; w += kMcBlock) bm[w] = 0u;
+    if (tid < 4) fl[tid] = 0;
+    // bounding box -> cell grid
+    {
+        double lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) lo[d] = hi[d] = pos[d];
+        for (int i = tid; i < n; i += kMcBlock)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = fmin(lo[d], pos[4 * i + d]);
+                hi[d] = fmax(hi[d], pos[4 * i + d]);
+            }
+        wave_minmax(lo, hi);
+        if (lane == 0)
+            for (int d = 0; d < 3; ++d) {
+                slo[d][wave] = lo[d];
+                shi[d][wave] = hi[d];
+            }
         __syncthreads();
+        if (tid == 0) {
+#pragma clang fp contract(off)
+            const double cs = sqrt(kNlRcut2) + 1e-6;
+            for (int d = 0; d < 3; ++d)
+                for (int w = 1; w < kMcWaves; ++w) {
+                    slo[d][0] = fmin(slo[d][0], slo[d][w]);
+                    shi[d][0] = fmax(shi[d][0], shi[d][w]);
+                }
+            int ncell = 1;
+            g->cs = cs;
+            for (int d = 0; d < 3; ++d) {
+                double l = slo[d][0] - cs;
+                double len = shi[d][0] - l + 2.0 * cs;
+                int k = (int)ceil(len / cs);
+                g->lo[d] = l;
+                g->nc[d] = k < 1 ? 1 : k;
+                ncell *= g->nc[d];
+            }
+            g->ncell = ncell;
+            if (ncell > b.max_cells) fl[0] |= 2;
+        }
+        wg_sync_global();
     }
-    if (t == 0) {
-        b.st[c].cur_e = kBetaSA + buf[0];
-        if (count_rebuild) b.st[c].rebuilds += 1;
-        if (b.st[c].status == CS_REBUILD) b.st[c].status = CS_RUN;
+    const Grid gr = *g;
+    const bool grid_ok = gr.ncell <= b.max_cells;
+    // cell of every atom
+    for (int i = tid; i < n; i += kMcBlock) {
+        int id = 0;
+        if (grid_ok) {
+            int cc[3];
+            for (int d = 0; d < 3; ++d) {
+                int v = (int)floor((pos[4 * i + d] - gr.lo[d]) / gr.cs);
+                cc[d] = min(gr.nc[d] - 1, max(0, v));
+            }
+            id = cc[0] + gr.nc[0] * (cc[1] + gr.nc[1] * cc[2]);
+        }
+        cell_of[i] = id;
+        atomicAdd(&cnt[id], 1);
     }
+    wg_sync_global();
+    // exclusive scan of the cell counts; counts reset to serve as cursors
+    {
+        const int ncell = min(gr.ncell, b.max_cells);
+        int carry = 0;
+        for (int base = 0; base < ncell; base += kMcBlock) {
+            int v = (base + tid < ncell) ? cnt[base + tid] : 0;
+            int x = v;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                int t = __shfl_up(x, o, kWave);
+                if (lane >= o) x += t;
+            }
+            if (lane == kWave - 1) sbuf[wave] = x;
+            __syncthreads();
+            int before = carry;
+            for (int w = 0; w < wave; ++w) before += sbuf[w];
+            if (base + tid < ncell) start[base + tid] = before + x - v;
+            int tot = 0;
+            for (int w = 0; w < kMcWaves; ++w) tot += sbuf[w];
+            __syncthreads();
+            carry += tot;
+        }
+        if (tid == 0) start[ncell] = carry;
+        __syncthreads();
+        for (int k = tid; k < ncell; k += kMcBlock) cnt[k] = 0;
+        wg_sync_global();
+    }
+    for (int i = tid; i < n; i += kMcBlock) {
+        int cl = cell_of[i];
+        cell_tmp[start[cl] + atomicAdd(&cnt[cl], 1)] = i;
+    }
+    wg_sync_global();
+    // ascending atom order inside each cell (the reference push_back order)
+    for (int i = tid; i < n; i += kMcBlock) {
+        int cl = cell_of[i];
+        int s0 = start[cl], s1 = start[cl + 1];
+        int r = 0;
+        for (int k = s0; k < s1; ++k) r += cell_tmp[k] < i;
+        atoms[s0 + r] = i;
+    }
+    __syncthreads();
+    // half rows, SASA rows, neighbour bitmap: one wavefront per atom
+    const int* exo = s.has_topo ? s.topo.excl_off : nullptr;
+    for (int i = wave; i < n && grid_ok; i += kMcWaves) {
+        const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+        const double ri = s.rstar[i];
+        int cl = cell_of[i];
+        int cx = cl % gr.nc[0], cy = (cl / gr.nc[0]) % gr.nc[1], cz = cl / (gr.nc[0] * gr.nc[1]);
+        int eb = 0, ee = 0;
+        if (exo) {
+            eb = exo[i];
+            ee = exo[i + 1];
+        }
+        uint32_t* rowh = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+        uint32_t* rows = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
+        uint32_t* bmi = bm + (size_t)i * W;
+        int nh = 0, ns = 0;
+        for (int dz = -1; dz <= 1; ++dz) {
+            int z = cz + dz;
+            if (z < 0 || z >= gr.nc[2]) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int y = cy + dy;
+                if (y < 0 || y >= gr.nc[1]) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int x = cx + dx;
+                    if (x < 0 || x >= gr.nc[0]) continue;
+                    int cc = x + gr.nc[0] * (y + gr.nc[1] * z);
+                    int e = start[cc + 1];
+                    for (int base = start[cc]; base < e; base += kWave) {
+                        int k = base + lane;
+                        bool keep = false, near = false;
+                        int j = 0;
+                        if (k < e) {
+                            j = atoms[k];
+                            if (j > i) {
+                                double d2 = dist2_exact(pi, pos + 4 * j);
+                                keep = d2 < kNlRcut2;
+                                double reach = ri + s.rstar[j] + 2.0 * kProbe + kSasaSkin;
+                                near = keep && d2 < reach * reach;
+                            }
+                        }
+                        uint64_t m = __ballot(keep);
+                        uint64_t mn = __ballot(near);
+                        uint64_t below = (1ull << lane) - 1ull;
+                        if (keep) {
+                            uint32_t v = (uint32_t)j;
+                            if (exo && in_sorted(s.topo.excl_idx, eb, ee, j)) v |= kExclBit;
+                            int slot = nh + __popcll(m & below);
+                            if (slot < b.cap_h) rowh[slot] = v;
+                            if (near) {
+                                int q = ns + __popcll(mn & below);
+                                if (q < b.cap_s) rows[q] = v;
+                            }
+                            atomicOr(&bmi[j >> 5], 1u << (j & 31));
+                            atomicOr(&bm[(size_t)j * W + (i >> 5)], 1u << (i & 31));
+                        }
+                        nh += __popcll(m);
+                        ns += __popcll(mn);
+                    }
+                }
+            }
+        }
+        if (lane == 0) {
+            b.nlh_cnt[R1(b, c) + i] = nh;
+            b.nls_cnt[R1(b, c) + i] = ns;
+            atomicMax(&fl[1], nh);
+            atomicMax(&fl[3], ns);
+            if (nh > b.cap_h || ns > b.cap_s) atomicOr(&fl[0], 1);
+        }
+    }
+    wg_sync_global();
+    // full rows in ascending index from the bitmap (excluded bit from the
+    // lower atom's exclusion list, as the half row marks it)
+    for (int i = wave; i < n; i += kMcWaves) {
+        const uint32_t* bmi = bm + (size_t)i * W;
+        uint32_t* rowf = b.nlf + (R1(b, c) + i) * (size_t)b.cap_f;
+        int total = 0;
+        for (int base = 0; base < W; base += kWave) {
+            uint32_t word = base + lane < W ? bmi[base + lane] : 0u;
+            int pc = __popc(word), x = pc;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                int t = __shfl_up(x, o, kWave);
+                if (lane >= o) x += t;
+            }
+            int slot = total + x - pc;
+            while (word) {
+                int bit = __ffs(word) - 1;
+                word &= word - 1u;
+                int j = ((base + lane) << 5) + bit;
+                uint32_t v = (uint32_t)j;
+                if (exo) {
+                    int lo = min(i, j), hi = max(i, j);
+                    if (in_sorted(s.topo.excl_idx, exo[lo], exo[lo + 1], hi)) v |= kExclBit;
+                }
+                if (slot < b.cap_f) rowf[slot] = v;
+                ++slot;
+            }
+            total += __shfl(x, kWave - 1, kWave);
+        }
+        if (lane == 0) {
+            b.nlf_cnt[R1(b, c) + i] = total;
+            atomicMax(&fl[2], total);
+            if (total > b.cap_f) atomicOr(&fl[0], 4);
+        }
+    }
+    // HCT sums over every half-row entry, both directions
+    for (int i = wave; i < n && grid_ok; i += kMcWaves) {
+        const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+        float ri = s.prm[i].y;
+        int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
+        const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+        double acc = 0.0;
+        for (int k = lane; k < m; k += kWave) {
+            int j = (int)(row[k] & kIdxMask);
+            float r2 = (float)dist2_exact(pi, pos + 4 * j), r = sqrtf(r2);
+            float rj = s.prm[j].y;
+            acc += (double)hct_f(r, r2, ri, rj);
+            atomicAdd(&sum[j], (double)hct_f(r, r2, rj, ri));
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            atomicAdd(&sum[i], acc);
+            b.half[R1(b, c) + i] = acc;
+        }
+    }
+    wg_sync_global();
+    for (int i = tid; i < n; i += kMcBlock) {
+        size_t o = R1(b, c) + i;
+        b.a[o] = 1.0 / fmax(1.0 / s.rstar[i] - 0.5 * sum[i], 2.0);
+        for (int d = 0; d < 4; ++d) ref[4 * i + d] = pos[4 * i + d];
+        dhalf[i] = 0.0;
+        flag[i] = 0;
+    }
+    __syncthreads();
+    // SASA fold of every atom over its SASA row (the incremental SASA change
+    // of a step is measured against these).  The fresh total energy the
+    // reference assigns to curE here (total_e, src/physics_engine.cpp:2509)
+    // is not needed by any later step: the chain keeps the sum of accepted dE
+    // since this build, and the total at the build positions is evaluated
+    // once, after the run, by the batched chain_energy kernels.
+    for (int i = wave; i < n && grid_ok; i += kMcWaves) {
+        const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+        double ri = (double)s.prm[i].y + kProbe;
+        int ms = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
+        int clamped;
+        double sa = sasa_fold_wave(pi, pos, s.prm, ri, b.nls + (R1(b, c) + i) * (size_t)b.cap_s, ms, &clamped);
+        if (lane == 0) b.part[5 * (R1(b, c) + i) + 4] = kGammaSA * fmax(0.0, sa);
+    }
+    if (tid == 0) {
+        ChainState* cs = b.st + c;
+        cs->de_since = 0.0;
+        if (count_rebuild) cs->rebuilds += 1;
+        int f = fl[0];
+        if (f & 2) cs->status = CS_GRID;
+        else if (f & 5) cs->status = CS_CAPACITY;
+        else cs->status = CS_RUN;
+    }
+    wg_sync_global();
 }
 
 // ───────────────────────────────── MC step ──────────────────────────────────
@@ -600,7 +535,20 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             t_prev = now;
         }
     };
+    // list builds: the first of the run (status CS_REBUILD at launch, not
+    // counted as a rebuild) and after every accepted move that drifted an
+    // atom past 1 A (needs_rebuild, :345-351)
+    __shared__ int pending;
+    if (tid == 0) pending = cs->status == CS_REBUILD ? 1 : 0;
+    __syncthreads();
     for (;;) {
+        if (pending) {
+            if (tid == 0) t_prev = wall_clock64();
+            chain_rebuild(b, s, c, pending - 1, dhalf, flag);
+            tick(7);
+            if (tid == 0) pending = 0;
+        }
+        if (cs->status != CS_RUN) break;
         if (tid == 0) t_prev = wall_clock64();
         // ---- move selection (lane 0 of wave 0; RNG order of the reference)
         if (tid == 0) {
@@ -1096,7 +1044,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     cs->status = CS_ERROR;
                 }
                 mv.accept = acc_ok ? 1 : 0;
-                if (acc_ok) cs->cur_e += dE;
+                if (acc_ok) cs->de_since += dE;
             }
             __syncthreads();
             tick(5);
@@ -1166,10 +1114,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             cs->steps_done = step + 1;
             cs->cursor = W.cur;
             if (cs->status == CS_ERROR) mv.stop = 1;
-            if (mv.rebuild) {
-                cs->status = CS_REBUILD;
-                mv.stop = 1;
-            }
+            if (mv.rebuild) pending = 2;
         }
         __syncthreads();
         if (mv.stop) break;
@@ -1177,27 +1122,119 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     if (tid == 0) cs->cursor = W.cur;
 }
 
+// ── curE of every chain after a run ───────────────────────────────────────
+// The reference resets curE to total_e at each list build
+// (src/physics_engine.cpp:2509) and adds every accepted dE.  The lists of a
+// chain's last build and its build positions (ref) are still held, so its
+// curE is total_e(ref) + the accepted dE since: Born radii from the half rows
+// at ref, pair terms within 12 A, SASA folds, dihedrals and disulfides, all
+// chains in one batched pass.
+__global__ void ce_clear(ChainBatch b) {
+    size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k < (size_t)b.nch * b.n) b.born_sum[k] = 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) ce_born(ChainBatch b, DevSys s) {
+    const int c = blockIdx.y;
+    const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (i >= b.n) return;
+    const int lane = lane_id();
+    const double* ref = b.ref + (size_t)c * 4 * b.n;
+    const double pi[3] = {ref[4 * i], ref[4 * i + 1], ref[4 * i + 2]};
+    float ri = s.prm[i].y;
+    int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
+    const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+    double* sum = b.born_sum + R1(b, c);
+    double acc = 0.0;
+    for (int k = lane; k < m; k += kWave) {
+        int j = (int)(row[k] & kIdxMask);
+        float r2 = (float)dist2_exact(pi, ref + 4 * j), r = sqrtf(r2);
+        float rj = s.prm[j].y;
+        acc += (double)hct_f(r, r2, ri, rj);
+        atomicAdd(&sum[j], (double)hct_f(r, r2, rj, ri));
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) atomicAdd(&sum[i], acc);
+}
+
+__global__ void ce_radii(ChainBatch b, DevSys s) {
+    size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (k >= (size_t)b.nch * b.n) return;
+    int i = (int)(k % b.n);
+    b.anew[k] = 1.0 / fmax(1.0 / s.rstar[i] - 0.5 * b.born_sum[k], 2.0);
+}
+
+__global__ void __launch_bounds__(kBlock) ce_terms(ChainBatch b, DevSys s) {
+    const int c = blockIdx.y;
+    const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    if (i >= b.n) return;
+    const int lane = lane_id();
+    const double* ref = b.ref + (size_t)c * 4 * b.n;
+    const double* a = b.anew + R1(b, c);
+    const double pi[3] = {ref[4 * i], ref[4 * i + 1], ref[4 * i + 2]};
+    float4 qi = atom4(s, a, i);
+    int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
+    const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+    double acc = 0.0;
+    for (int k = lane; k < m; k += kWave) {
+        uint32_t e = row[k];
+        if (e & kExclBit) continue;
+        int j = (int)(e & kIdxMask);
+        double r2 = dist2_exact(pi, ref + 4 * j);
+        if (r2 > kPairCut2) continue;
+        acc += pair_energy(r2, qi, atom4(s, a, j), s.rstar[i], s.rstar[j], s.eps64[i], s.eps64[j]);
+    }
+    acc = wave_sum(acc);
+    double ri = (double)s.prm[i].y + kProbe;
+    int clamped;
+    double sa = sasa_fold_wave(pi, ref, s.prm, ri, row, m, &clamped);
+    if (lane == 0) {
+        b.part[5 * (R1(b, c) + i)] = acc;
+        b.part[5 * (R1(b, c) + i) + 1] = kGammaSA * fmax(0.0, sa);
+    }
+}
+
+__global__ void ce_total(ChainBatch b, DevSys s) {
+    const int c = blockIdx.x;
+    __shared__ double buf[kBlock];
+    const int t = threadIdx.x;
+    const double* ref = b.ref + (size_t)c * 4 * b.n;
+    double acc = 0.0;
+    for (int i = t; i < b.n; i += kBlock) acc += b.part[5 * (R1(b, c) + i)] + b.part[5 * (R1(b, c) + i) + 1];
+    if (s.has_topo) {
+        for (int k = t; k < s.topo.n_dih; k += kBlock) {
+            double phi = dih_angle_d(ref, s.topo.dih_a[k], s.topo.dih_b[k], s.topo.dih_c[k], s.topo.dih_d[k]);
+            for (int m = s.topo.dih_toff[k]; m < s.topo.dih_toff[k + 1]; ++m)
+                acc += s.topo.dih_v2[m] * (1.0 + cos((double)s.topo.dih_n[m] * phi - s.topo.dih_gamma[m]));
+        }
+        for (int k = t; k < s.topo.n_ss; k += kBlock) {
+            int i = s.topo.ss_i[k], j = s.topo.ss_j[k];
+            double dr = sqrt(dist2_exact(ref + 4 * i, ref + 4 * j)) - kR0ss;
+            acc += kKss * dr * dr;
+        }
+    }
+    buf[t] = acc;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (t < h) buf[t] += buf[t + h];
+        __syncthreads();
+    }
+    if (t == 0) b.st[c].cur_e = kBetaSA + buf[0] + b.st[c].de_since;
+}
+
 inline unsigned blocks_for(int n, int per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
 
-hipError_t launch_chain_rebuild(ChainBatch& b, const DevSys& s, int L, int count, hipStream_t st) {
-    if (L <= 0) return hipSuccess;
-    const double cs = std::sqrt(kNlRcut2) + 1e-6;
-    dim3 g1(1, L), ga(blocks_for(b.n, kBlock), L), gw(blocks_for(b.n, kWavesPerBlock), L);
-    rb_clear<<<dim3(16, L), kBlock, 0, st>>>(b);
-    rb_bbox<<<g1, kBlock, 0, st>>>(b, cs);
-    rb_cell_assign<<<ga, kBlock, 0, st>>>(b);
-    rb_cell_scan<<<g1, 1024, 0, st>>>(b);
-    rb_cell_place<<<ga, kBlock, 0, st>>>(b);
-    rb_cell_rank<<<ga, kBlock, 0, st>>>(b);
-    rb_nl_fill<<<gw, kBlock, 0, st>>>(b, s);
-    rb_full_sort<<<dim3(b.n, L), kSortBlock, sizeof(uint32_t) * b.cap_f, st>>>(b);
-    rb_born_sum<<<gw, kBlock, 0, st>>>(b, s);
-    rb_born_fin<<<ga, kBlock, 0, st>>>(b, s);
-    rb_pair<<<gw, kBlock, 0, st>>>(b, s);
-    rb_sasa<<<ga, kBlock, 0, st>>>(b, s);
-    rb_total<<<g1, kBlock, 0, st>>>(b, s, count);
+hipError_t launch_chain_energy(ChainBatch& b, const DevSys& s, hipStream_t st) {
+    const size_t N = (size_t)b.nch * b.n;
+    const unsigned g1 = (unsigned)((N + kBlock - 1) / kBlock);
+    dim3 gw(blocks_for(b.n, kWavesPerBlock), b.nch);
+    ce_clear<<<g1, kBlock, 0, st>>>(b);
+    ce_born<<<gw, kBlock, 0, st>>>(b, s);
+    ce_radii<<<g1, kBlock, 0, st>>>(b, s);
+    ce_terms<<<gw, kBlock, 0, st>>>(b, s);
+    ce_total<<<b.nch, kBlock, 0, st>>>(b, s);
     return hipGetLastError();
 }
 

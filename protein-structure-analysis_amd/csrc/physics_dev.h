@@ -173,6 +173,103 @@ __device__ __forceinline__ double pair_delta(double r2o, double r2n, float4 po_i
            lj_hc(r2o, po_i, po_j, Ri, Rj, ei, ej);
 }
 
+// ── SASA fold, one wavefront per atom row ───────────────────────────────────
+// sasa_nonpolar, src/physics_engine.cpp:2068-2083: for the row's entries in
+// list order, a contact (r < dc = R_i + R_j + 2 probe, r from the FP32 sqrt of
+// the FP64 d^2) removes min(0.85 sa, pi R_i (dc - r)).  Lanes test 64 entries
+// at once; the order-dependent fold then walks the contacts of the chunk in
+// order through wave shuffles, so the fold is bit-identical to a serial one.
+struct SasaContact {
+    bool hit;
+    int j;
+    double cap;
+};
+
+__device__ __forceinline__ SasaContact sasa_contact(const double* pi, const double* pos, const float4* prm,
+                                                    double ri, double k2, uint32_t e) {
+    SasaContact c{false, (int)(e & kIdxMask), 0.0};
+    double dc = ri + (double)prm[c.j].y + kProbe;
+    double r2 = dist2_exact(pi, pos + 4 * c.j);
+    if (r2 < dc * dc) {
+        double r = (double)sqrtf((float)r2);
+        if (r < dc) {
+            c.hit = true;
+            c.cap = k2 * (dc - r);
+        }
+    }
+    return c;
+}
+
+// Folds the row; returns the final area (wave-uniform) and the number of
+// clamped contacts.
+__device__ inline double sasa_fold_wave(const double* pi, const double* pos, const float4* prm, double ri,
+                                        const uint32_t* row, int m, int* n_clamped) {
+    const int lane = lane_id();
+    const double k2 = 2.0 * M_PI * ri * ri / (2.0 * ri);
+    double sa = 4.0 * M_PI * ri * ri;
+    int clamped = 0;
+    for (int base = 0; base < m; base += kWave) {
+        SasaContact c{false, 0, 0.0};
+        if (base + lane < m) c = sasa_contact(pi, pos, prm, ri, k2, row[base + lane]);
+        uint64_t mask = __ballot(c.hit);
+        while (mask) {
+            int b = __ffsll((unsigned long long)mask) - 1;
+            mask &= mask - 1;
+            double cap = __shfl(c.cap, b, kWave);
+            if (sa * 0.85 < cap) {
+                sa -= sa * 0.85;
+                ++clamped;
+            } else {
+                sa -= cap;
+            }
+        }
+    }
+    *n_clamped = clamped;
+    return sa;
+}
+
+// Adjoint of the fold for a wavefront: each unclamped contact carries
+// dE/dr = gamma pi R_i 0.15^(clamped contacts after it); adds the radial
+// forces on j into G (atomics) and returns this lane's share of atom i's.
+__device__ inline void sasa_adjoint_wave(const double* pi, const double* pos, const float4* prm, double ri,
+                                         const uint32_t* row, int m, int n_clamped, double* G, double* gi) {
+    const int lane = lane_id();
+    const double k2 = 2.0 * M_PI * ri * ri / (2.0 * ri);
+    double sa = 4.0 * M_PI * ri * ri;
+    int before = 0;
+    for (int base = 0; base < m; base += kWave) {
+        SasaContact c{false, 0, 0.0};
+        if (base + lane < m) c = sasa_contact(pi, pos, prm, ri, k2, row[base + lane]);
+        uint64_t mask = __ballot(c.hit), cl = 0;
+        uint64_t walk = mask;
+        while (walk) {
+            int b = __ffsll((unsigned long long)walk) - 1;
+            walk &= walk - 1;
+            double cap = __shfl(c.cap, b, kWave);
+            if (sa * 0.85 < cap) {
+                sa -= sa * 0.85;
+                cl |= 1ull << b;
+            } else {
+                sa -= cap;
+            }
+        }
+        if (c.hit && !((cl >> lane) & 1ull)) {
+            int after = n_clamped - before - __popcll(cl & ((1ull << lane) - 1ull));
+            double r2 = dist2_exact(pi, pos + 4 * c.j);
+            double sc = kGammaSA * k2 * pow(0.15, (double)after) / sqrt(r2);
+            double u0 = (pi[0] - pos[4 * c.j]) * sc, u1 = (pi[1] - pos[4 * c.j + 1]) * sc,
+                   u2 = (pi[2] - pos[4 * c.j + 2]) * sc;
+            gi[0] += u0;
+            gi[1] += u1;
+            gi[2] += u2;
+            atomicAdd(&G[3 * c.j], -u0);
+            atomicAdd(&G[3 * c.j + 1], -u1);
+            atomicAdd(&G[3 * c.j + 2], -u2);
+        }
+        before += __popcll(cl);
+    }
+}
+
 // ── dihedral / disulfide (FP64) ─────────────────────────────────────────────
 // dihedral_angle :1871-1885 (atan2 of the reference's non-IUPAC pair),
 // dihedral_e :1912-1921, ss_e :1965-1974.
