@@ -54,6 +54,8 @@ void dev_free(T*& p) {
 void free_topo(DevTopo& t) {
     dev_free(t.excl_off);
     dev_free(t.excl_idx);
+    dev_free(t.exs_off);
+    dev_free(t.exs_idx);
     dev_free(t.rb_i);
     dev_free(t.rb_j);
     dev_free(t.rb_kind);
@@ -182,6 +184,24 @@ void upload_topo(DevTopo& d, const alma_topo* t, int n) {
     free_topo(d);
     dev_upload(&d.excl_off, t->excl_off, (size_t)n + 1);
     dev_upload(&d.excl_idx, t->excl_idx, (size_t)t->excl_off[n]);
+    {
+        // both directions of every exclusion, ascending per atom
+        std::vector<std::vector<int>> rows(n);
+        for (int i = 0; i < n; ++i)
+            for (int k = t->excl_off[i]; k < t->excl_off[i + 1]; ++k) {
+                rows[i].push_back(t->excl_idx[k]);
+                rows[t->excl_idx[k]].push_back(i);
+            }
+        std::vector<int> off(n + 1, 0), idx;
+        for (int i = 0; i < n; ++i) {
+            std::sort(rows[i].begin(), rows[i].end());
+            rows[i].erase(std::unique(rows[i].begin(), rows[i].end()), rows[i].end());
+            idx.insert(idx.end(), rows[i].begin(), rows[i].end());
+            off[i + 1] = (int)idx.size();
+        }
+        dev_upload(&d.exs_off, off.data(), off.size());
+        dev_upload(&d.exs_idx, idx.data(), idx.size());
+    }
     d.n_rot = t->n_rot;
     d.n_dih = t->n_dih;
     d.n_ss = t->n_ss;
@@ -322,6 +342,8 @@ void free_chains(ChainBatch& b) {
     dev_free(b.nlf);
     dev_free(b.nls);
     dev_free(b.bm);
+    dev_free(b.stage);
+    dev_free(b.stage_idx);
     dev_free(b.nlh_cnt);
     dev_free(b.nlf_cnt);
     dev_free(b.nls_cnt);
@@ -351,7 +373,7 @@ void alloc_rows(ChainBatch& b) {
     dev_alloc(&b.nlh, rows * b.cap_h);
     dev_alloc(&b.nlf, rows * b.cap_f);
     dev_alloc(&b.nls, rows * b.cap_s);
-    dev_alloc(&b.bm, rows * (size_t)((b.n + 31) / 32));
+    dev_alloc(&b.bm, (size_t)b.nch * 2 * 16 * (size_t)((b.n + 31) / 32));   // per-wavefront row bitmaps + counts
 }
 
 void alloc_chains(ChainBatch& b, const DevSys& s, int nch, int trace_steps) {
@@ -385,6 +407,8 @@ void alloc_chains(ChainBatch& b, const DevSys& s, int nch, int trace_steps) {
     dev_alloc(&b.st, nch);
     dev_alloc(&b.list, nch);
     dev_alloc(&b.seg, N * 4);
+    dev_alloc(&b.stage, N);
+    dev_alloc(&b.stage_idx, N);
     dev_alloc(&b.aff, N);
     HIP_TRY(hipMemset(b.dhalf, 0, sizeof(double) * N));
     HIP_TRY(hipMemset(b.flag, 0, sizeof(int) * N));
@@ -572,6 +596,12 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
                      x.clk[4] / 100.0 / x.steps_done, x.clk[5] / 100.0 / x.steps_done,
                      x.clk[6] / 100.0 / x.steps_done, tot / 100.0 / x.steps_done,
                      x.rebuilds ? x.clk[7] / 100.0 / x.rebuilds : 0.0, x.rebuilds);
+        long long nb = x.rebuilds + 1;
+        std::fprintf(stderr, "[alma mc] list build us: grid %.1f cells %.1f fill %.1f born %.1f sasa %.1f | per-wave sums: "
+                     "atom setup %.1f candidate scan %.1f full-row conversion %.1f\n",
+                     x.rclk[0] / 100.0 / nb, x.rclk[1] / 100.0 / nb, x.rclk[2] / 100.0 / nb, x.rclk[4] / 100.0 / nb,
+                     x.rclk[5] / 100.0 / nb, x.rclk[6] / 100.0 / nb / 16, x.rclk[7] / 100.0 / nb / 16,
+                     x.rclk[3] / 100.0 / nb / 16);
     }
     // results
     std::vector<double> p4(4 * (size_t)n);

@@ -44,6 +44,8 @@ struct DevTopo {
     int n_rot = 0, n_dih = 0, n_ss = 0, n_cp = 0, n_csp = 0;
     int* excl_off = nullptr;
     int* excl_idx = nullptr;
+    int* exs_off = nullptr;        // symmetric exclusions: both directions, ascending
+    int* exs_idx = nullptr;
     int* rb_i = nullptr;
     int* rb_j = nullptr;
     int* rb_kind = nullptr;
@@ -116,6 +118,7 @@ struct ChainState {
     int acc_win, tot_win;
     int mode;
     long long clk[8];    // per-phase clock ticks (select, move, segments, pass A, radii, pass B, commit, list rebuild)
+    long long rclk[8];   // list-build phases (clear+grid, cells, fill, full rows, Born, SASA)
     long long steps_by_kind[3];
     // algorithmic work census: cross pairs (HCT change), in-cutoff pair
     // evaluations (old + new), SASA row entries folded, moved atoms,
@@ -142,7 +145,9 @@ struct ChainBatch {
     uint32_t* nlh = nullptr;  // nch * n * cap_h half rows, cell-scan order, excl bit
     uint32_t* nlf = nullptr;  // nch * n * cap_f full rows, ascending index, excl bit
     uint32_t* nls = nullptr;  // nch * n * cap_s half rows within SASA reach + skin
-    uint32_t* bm = nullptr;   // nch * n * ceil(n/32) neighbour bitmap (list rebuild scratch)
+    uint32_t* bm = nullptr;   // nch * 16 * ceil(n/32) per-wavefront row bitmaps (list build, large n)
+    float4* stage = nullptr;  // nch * n cell-ordered build coordinates (when LDS cannot hold them)
+    int* stage_idx = nullptr; // nch * n
     int* nlh_cnt = nullptr;
     int* nlf_cnt = nullptr;
     int* nls_cnt = nullptr;

@@ -35,6 +35,7 @@ constexpr double kSasaTrial = 1.5;   // max trial displacement served by it
 constexpr int kLdsRuns = 16;         // moved-set runs cached in LDS
 constexpr int kSegRuns = 3;          // runs handled by the segment path
 constexpr int kLdsAtoms = 11264;     // systems up to this size keep per-step atom state in LDS
+constexpr int kLdsRebuild = 5632;    // ... and stage the list build in LDS (about 24 B per atom)
 
 __device__ __forceinline__ double* P4(const ChainBatch& b, double* base, int c) {
     return base + (size_t)c * 4 * b.n;
@@ -79,8 +80,267 @@ __device__ __forceinline__ void wave_minmax(double* lo, double* hi) {
         }
 }
 
+// LDS-typed views of the list-build stage (ds_* instructions instead of
+// flat ones when the stage lives in LDS)
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) const int lds_ci32;
+typedef __attribute__((address_space(3))) const float lds_cf32;
+
+// Completes this wavefront's outstanding LDS / memory operations before its
+// lanes read what other lanes wrote (workgroup-scope fences wait for them).
+__device__ __forceinline__ void wave_sync_mem() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+// The same for data exchanged through LDS only: wait for this wavefront's
+// LDS operations (lgkmcnt(0)), leave its global stores in flight.
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Exclusion test against atom i's symmetric exclusion row cached in LDS
+// (ex[0..ne), ascending; rows longer than kExRow fall back to a binary
+// search in global memory).
+constexpr int kExRow = 64;
+constexpr int kLdsCells = 1024;   // cell-start tables up to this size are read from LDS
+__device__ __forceinline__ bool excluded_row(const int* ex, int ne, const DevTopo& T, int i, int j) {
+    if (ne > kExRow) return in_sorted(T.exs_idx, T.exs_off[i], T.exs_off[i + 1], j);
+    if (ne == 0 || j < ex[0] || j > ex[ne - 1]) return false;
+    for (int t = 0; t < ne; ++t)
+        if (ex[t] == j) return true;
+    return false;
+}
+
+__device__ __forceinline__ int load_ex_row(const DevTopo& T, bool has_topo, int i, int* ex) {
+    if (!has_topo) return 0;
+    const int e0 = T.exs_off[i], ne = T.exs_off[i + 1] - e0;
+    if (ne <= kExRow)
+        for (int t = lane_id(); t < ne; t += kWave) ex[t] = T.exs_idx[e0 + t];
+    wave_sync_lds();
+    return ne;
+}
+
+// One wavefront per atom i over the staged coordinates: every partner j != i
+// within the list radius sets bit j of the wavefront's row bitmap, partners
+// j > i are appended to the half row in scan order (and to the SASA row when
+// within reach); the bitmap then yields the full row in ascending index.
+// The FP32 distance (relative coordinates below ~100 A: |error of d^2| <
+// 1e-3 A^2) decides every pair except those within 0.01 A^2 of the 196 A^2
+// radius, which take the reference's FP64 test; the SASA-row reach carries
+// the same margin.  kLds: stage, cell starts and row bitmap in LDS, so the
+// candidate loop issues no global loads and never waits on its stores.
+template <bool kLds>
+__device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, int c, const Grid& gr,
+                                          const int* cell_of, const int* start, const float4* cp, const int* ci,
+                                          uint32_t* rb, int* pref, int* ex, int& max_h, int& max_s, int& max_f,
+                                          int& ovf) {
+    const int n = b.n;
+    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const int W = (n + 31) / 32;
+    const double* pos = b.pos + (size_t)c * 4 * n;
+    const DevTopo& T = s.topo;
+    const bool topo = s.has_topo;
+    const float fcs = (float)gr.cs;
+    long long t_scan = 0, t_conv = 0, t_pre = 0;
+    for (int i = wave; i < n; i += kMcWaves) {
+        long long ta = wall_clock64();
+        const double* pid = pos + 4 * i;
+        const float pix = (float)(pid[0] - gr.lo[0]), piy = (float)(pid[1] - gr.lo[1]),
+                    piz = (float)(pid[2] - gr.lo[2]);
+        const float ri = s.prm[i].y;
+        const int ne = load_ex_row(T, topo, i, ex);
+        for (int w = lane; w < W; w += kWave) {
+            if (kLds)
+                ((lds_u32*)rb)[w] = 0u;
+            else
+                rb[w] = 0u;
+        }
+        if (kLds)
+            wave_sync_lds();
+        else
+            wave_sync_mem();
+        const int cl = cell_of[i];
+        const int cx = cl % gr.nc[0], cy = (cl / gr.nc[0]) % gr.nc[1], cz = cl / (gr.nc[0] * gr.nc[1]);
+        // position inside the cell, for skipping neighbour cells whose box
+        // lies beyond the list radius (FP32 margin 0.1 A^2)
+        const float ox = pix - cx * fcs, oy = piy - cy * fcs, oz = piz - cz * fcs;
+        uint32_t* rowh = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+        uint32_t* rows = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
+        int nh = 0, ns = 0;
+        long long tb = wall_clock64();
+        t_pre += tb - ta;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int z = cz + dz;
+            if (z < 0 || z >= gr.nc[2]) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int y = cy + dy;
+                if (y < 0 || y >= gr.nc[1]) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int x = cx + dx;
+                    if (x < 0 || x >= gr.nc[0]) continue;
+                    {
+                        float gx = dx < 0 ? ox : dx > 0 ? fcs - ox : 0.0f;
+                        float gy = dy < 0 ? oy : dy > 0 ? fcs - oy : 0.0f;
+                        float gz = dz < 0 ? oz : dz > 0 ? fcs - oz : 0.0f;
+                        if (gx * gx + gy * gy + gz * gz > (float)kNlRcut2 + 0.1f) continue;
+                    }
+                    const int cc = x + gr.nc[0] * (y + gr.nc[1] * z);
+                    int b0, e;
+                    if (kLds) {
+                        b0 = ((lds_ci32*)start)[cc];
+                        e = ((lds_ci32*)start)[cc + 1];
+                    } else {
+                        b0 = start[cc];
+                        e = start[cc + 1];
+                    }
+                    for (int base = b0; base < e; base += kWave) {
+                        const int k = base + lane;
+                        bool keep = false, ambiguous = false;
+                        float d2 = 0.0f, rj = 0.0f;
+                        int j = i;
+                        if (k < e) {
+                            float4 q;
+                            if (kLds) {
+                                j = ((lds_ci32*)ci)[k];
+                                lds_cf32* qp = (lds_cf32*)cp + 4 * k;
+                                q = make_float4(qp[0], qp[1], qp[2], qp[3]);
+                            } else {
+                                j = ci[k];
+                                q = cp[k];
+                            }
+                            float ddx = q.x - pix, ddy = q.y - piy, ddz = q.z - piz;
+                            d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            rj = q.w;
+                            keep = j != i && d2 < (float)kNlRcut2 - 0.01f;
+                            ambiguous = j != i && !keep && d2 < (float)kNlRcut2 + 0.01f;
+                        }
+                        if (__ballot(ambiguous)) {   // rare: FP64 test of the reference
+                            if (ambiguous) keep = dist2_exact(pid, pos + 4 * j) < kNlRcut2;
+                        }
+                        if (keep) {
+                            if (kLds)
+                                __hip_atomic_fetch_or((lds_u32*)rb + (j >> 5), 1u << (j & 31), __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                            else
+                                atomicOr(&rb[j >> 5], 1u << (j & 31));
+                        }
+                        const bool up = keep && j > i;
+                        const uint64_t m = __ballot(up);
+                        if (m) {
+                            float reach = ri + rj + (float)(2.0 * kProbe + kSasaSkin);
+                            const bool near = up && d2 < reach * reach + 0.01f;
+                            const uint64_t mn = __ballot(near);
+                            const uint64_t below = (1ull << lane) - 1ull;
+                            if (up) {
+                                uint32_t v = (uint32_t)j;
+                                if (topo && excluded_row(ex, ne, T, i, j)) v |= kExclBit;
+                                int slot = nh + __popcll(m & below);
+                                if (slot < b.cap_h) rowh[slot] = v;
+                                if (near) {
+                                    int q = ns + __popcll(mn & below);
+                                    if (q < b.cap_s) rows[q] = v;
+                                }
+                            }
+                            nh += __popcll(m);
+                            ns += __popcll(mn);
+                        }
+                    }
+                }
+            }
+        }
+        long long tc = wall_clock64();
+        t_scan += tc - tb;
+        // full row in ascending index from the row bitmap: inclusive prefix
+        // counts of the words, then output slot t (lane-strided, so the
+        // stores coalesce) finds its word by binary search and its bit by a
+        // five-step select
+        if (kLds)
+            wave_sync_lds();
+        else
+            __threadfence();
+        uint32_t* rowf = b.nlf + (R1(b, c) + i) * (size_t)b.cap_f;
+        int total = 0;
+        for (int wb = 0; wb < W; wb += kWave) {
+            uint32_t word = 0u;
+            if (wb + lane < W)
+                word = kLds ? ((lds_u32*)rb)[wb + lane]
+                            : __hip_atomic_load(rb + wb + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int xs = __popc(word);
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                int t = __shfl_up(xs, o, kWave);
+                if (lane >= o) xs += t;
+            }
+            if (wb + lane < W) {
+                if (kLds)
+                    ((__attribute__((address_space(3))) int*)pref)[wb + lane] = total + xs;
+                else
+                    pref[wb + lane] = total + xs;
+            }
+            total += __shfl(xs, kWave - 1, kWave);
+        }
+        if (kLds)
+            wave_sync_lds();
+        else
+            __threadfence();
+        for (int t = lane; t < total && t < b.cap_f; t += kWave) {
+            int lo = 0, hi = W - 1;   // first word whose inclusive count exceeds t
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                int pm = kLds ? ((lds_ci32*)pref)[mid] : __hip_atomic_load(pref + mid, __ATOMIC_RELAXED,
+                                                                             __HIP_MEMORY_SCOPE_AGENT);
+                if (pm > t)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            int before = 0;
+            if (lo > 0)
+                before = kLds ? ((lds_ci32*)pref)[lo - 1]
+                              : __hip_atomic_load(pref + lo - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t x = kLds ? ((lds_u32*)rb)[lo] : __hip_atomic_load(rb + lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int r = t - before, bit = 0;
+#pragma unroll
+            for (int h = 16; h > 0; h >>= 1) {   // r-th set bit of x
+                int lowc = __popc(x & ((1u << h) - 1u));
+                if (r >= lowc) {
+                    r -= lowc;
+                    x >>= h;
+                    bit += h;
+                }
+            }
+            int j = (lo << 5) + bit;
+            uint32_t v = (uint32_t)j;
+            if (topo && excluded_row(ex, ne, T, i, j)) v |= kExclBit;
+            rowf[t] = v;
+        }
+        t_conv += wall_clock64() - tc;
+        if (lane == 0) {
+            b.nlh_cnt[R1(b, c) + i] = nh;
+            b.nls_cnt[R1(b, c) + i] = ns;
+            b.nlf_cnt[R1(b, c) + i] = total;
+            max_h = max(max_h, nh);
+            max_s = max(max_s, ns);
+            max_f = max(max_f, total);
+            if (nh > b.cap_h || ns > b.cap_s) ovf |= 1;
+            if (total > b.cap_f) ovf |= 4;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) {
+        atomicAdd((unsigned long long*)&b.st[c].rclk[6], (unsigned long long)t_pre);
+        atomicAdd((unsigned long long*)&b.st[c].rclk[7], (unsigned long long)t_scan);
+        atomicAdd((unsigned long long*)&b.st[c].rclk[3], (unsigned long long)t_conv);
+    }
+}
+
+// stage: n float4 (cell-ordered coordinates relative to the grid origin in
+// FP32, w = R*) followed by n ints (atom of each cell slot); LDS when the
+// chain's atoms fit (kLdsRebuild), global scratch otherwise
 __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int count_rebuild, double* dhalf,
-                              int* flag) {
+                              int* flag, float4* cp, int* ci, uint32_t* rbits, bool rbits_lds) {
     const int n = b.n;
     const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
     const int W = (n + 31) / 32;
@@ -91,17 +351,28 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
     int* cnt = b.cell_cnt + (size_t)c * (b.max_cells + 1);
     int* start = b.cell_start + (size_t)c * (b.max_cells + 1);
     int* cell_of = b.cell_of + R1(b, c);
-    int* cell_tmp = b.cell_tmp + R1(b, c);
-    int* atoms = b.cell_atoms + R1(b, c);
-    uint32_t* bm = b.bm + R1(b, c) * (size_t)W;
+    int* tmp = (int*)cp;   // slot scratch before the coordinates are staged
+    uint32_t* rb = rbits + (size_t)wave * W;                          // this wavefront's row bitmap
+    int* pref = (int*)(rbits + (size_t)(kMcWaves + wave) * W);        // ... and its word prefix counts
     double* sum = b.born_sum + R1(b, c);
+    const DevTopo& T = s.topo;
     __shared__ double slo[3][kMcWaves], shi[3][kMcWaves];
     __shared__ int sbuf[kMcBlock];
+    __shared__ int sex[kMcWaves][kExRow];
+    __shared__ int sstart[kLdsCells + 1];
+    int* ex = sex[wave];
+    long long t0 = tid == 0 ? wall_clock64() : 0;
+    auto rtick = [&](int ph) {
+        if (tid == 0) {
+            long long now = wall_clock64();
+            b.st[c].rclk[ph] += now - t0;
+            t0 = now;
+        }
+    };
 
     // clear
     for (int i = tid; i < n; i += kMcBlock) sum[i] = 0.0;
     for (int k = tid; k <= b.max_cells; k += kMcBlock) cnt[k] = 0;
-    for (size_t w = tid; w < (size_t)n * W; w += kMcBlock) bm[w] = 0u;
     if (tid < 4) fl[tid] = 0;
     // bounding box -> cell grid
     {
@@ -142,6 +413,7 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
         }
         wg_sync_global();
     }
+    rtick(0);
     const Grid gr = *g;
     const bool grid_ok = gr.ncell <= b.max_cells;
     // cell of every atom
@@ -185,10 +457,14 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
         __syncthreads();
         for (int k = tid; k < ncell; k += kMcBlock) cnt[k] = 0;
         wg_sync_global();
+        if (ncell < kLdsCells)
+            for (int k = tid; k <= ncell; k += kMcBlock) sstart[k] = start[k];
+        __syncthreads();
     }
+    const bool lds_cells = min(gr.ncell, b.max_cells) < kLdsCells;
     for (int i = tid; i < n; i += kMcBlock) {
         int cl = cell_of[i];
-        cell_tmp[start[cl] + atomicAdd(&cnt[cl], 1)] = i;
+        tmp[start[cl] + atomicAdd(&cnt[cl], 1)] = i;
     }
     wg_sync_global();
     // ascending atom order inside each cell (the reference push_back order)
@@ -196,115 +472,35 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
         int cl = cell_of[i];
         int s0 = start[cl], s1 = start[cl + 1];
         int r = 0;
-        for (int k = s0; k < s1; ++k) r += cell_tmp[k] < i;
-        atoms[s0 + r] = i;
+        for (int k = s0; k < s1; ++k) r += tmp[k] < i;
+        ci[s0 + r] = i;
     }
     __syncthreads();
-    // half rows, SASA rows, neighbour bitmap: one wavefront per atom
-    const int* exo = s.has_topo ? s.topo.excl_off : nullptr;
-    for (int i = wave; i < n && grid_ok; i += kMcWaves) {
-        const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
-        const double ri = s.rstar[i];
-        int cl = cell_of[i];
-        int cx = cl % gr.nc[0], cy = (cl / gr.nc[0]) % gr.nc[1], cz = cl / (gr.nc[0] * gr.nc[1]);
-        int eb = 0, ee = 0;
-        if (exo) {
-            eb = exo[i];
-            ee = exo[i + 1];
-        }
-        uint32_t* rowh = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
-        uint32_t* rows = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
-        uint32_t* bmi = bm + (size_t)i * W;
-        int nh = 0, ns = 0;
-        for (int dz = -1; dz <= 1; ++dz) {
-            int z = cz + dz;
-            if (z < 0 || z >= gr.nc[2]) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                int y = cy + dy;
-                if (y < 0 || y >= gr.nc[1]) continue;
-                for (int dx = -1; dx <= 1; ++dx) {
-                    int x = cx + dx;
-                    if (x < 0 || x >= gr.nc[0]) continue;
-                    int cc = x + gr.nc[0] * (y + gr.nc[1] * z);
-                    int e = start[cc + 1];
-                    for (int base = start[cc]; base < e; base += kWave) {
-                        int k = base + lane;
-                        bool keep = false, near = false;
-                        int j = 0;
-                        if (k < e) {
-                            j = atoms[k];
-                            if (j > i) {
-                                double d2 = dist2_exact(pi, pos + 4 * j);
-                                keep = d2 < kNlRcut2;
-                                double reach = ri + s.rstar[j] + 2.0 * kProbe + kSasaSkin;
-                                near = keep && d2 < reach * reach;
-                            }
-                        }
-                        uint64_t m = __ballot(keep);
-                        uint64_t mn = __ballot(near);
-                        uint64_t below = (1ull << lane) - 1ull;
-                        if (keep) {
-                            uint32_t v = (uint32_t)j;
-                            if (exo && in_sorted(s.topo.excl_idx, eb, ee, j)) v |= kExclBit;
-                            int slot = nh + __popcll(m & below);
-                            if (slot < b.cap_h) rowh[slot] = v;
-                            if (near) {
-                                int q = ns + __popcll(mn & below);
-                                if (q < b.cap_s) rows[q] = v;
-                            }
-                            atomicOr(&bmi[j >> 5], 1u << (j & 31));
-                            atomicOr(&bm[(size_t)j * W + (i >> 5)], 1u << (i & 31));
-                        }
-                        nh += __popcll(m);
-                        ns += __popcll(mn);
-                    }
-                }
-            }
-        }
-        if (lane == 0) {
-            b.nlh_cnt[R1(b, c) + i] = nh;
-            b.nls_cnt[R1(b, c) + i] = ns;
-            atomicMax(&fl[1], nh);
-            atomicMax(&fl[3], ns);
-            if (nh > b.cap_h || ns > b.cap_s) atomicOr(&fl[0], 1);
-        }
+    // cell-ordered FP32 coordinates relative to the grid origin
+    for (int k = tid; k < n; k += kMcBlock) {
+        int j = ci[k];
+        cp[k] = make_float4((float)(pos[4 * j] - gr.lo[0]), (float)(pos[4 * j + 1] - gr.lo[1]),
+                            (float)(pos[4 * j + 2] - gr.lo[2]), s.prm[j].y);
+    }
+    __syncthreads();
+    rtick(1);
+    const bool topo = s.has_topo;
+    int max_h = 0, max_s = 0, max_f = 0, ovf = 0;
+    if (grid_ok) {
+        if (rbits_lds && lds_cells)
+            fill_rows<true>(b, s, c, gr, cell_of, sstart, cp, ci, rb, pref, ex, max_h, max_s, max_f, ovf);
+        else
+            fill_rows<false>(b, s, c, gr, cell_of, start, cp, ci, rb, pref, ex, max_h, max_s, max_f, ovf);
+    }
+    (void)topo;
+    if (lane == 0) {
+        atomicMax(&fl[1], max_h);
+        atomicMax(&fl[2], max_f);
+        atomicMax(&fl[3], max_s);
+        if (ovf) atomicOr(&fl[0], ovf);
     }
     wg_sync_global();
-    // full rows in ascending index from the bitmap (excluded bit from the
-    // lower atom's exclusion list, as the half row marks it)
-    for (int i = wave; i < n; i += kMcWaves) {
-        const uint32_t* bmi = bm + (size_t)i * W;
-        uint32_t* rowf = b.nlf + (R1(b, c) + i) * (size_t)b.cap_f;
-        int total = 0;
-        for (int base = 0; base < W; base += kWave) {
-            uint32_t word = base + lane < W ? bmi[base + lane] : 0u;
-            int pc = __popc(word), x = pc;
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                int t = __shfl_up(x, o, kWave);
-                if (lane >= o) x += t;
-            }
-            int slot = total + x - pc;
-            while (word) {
-                int bit = __ffs(word) - 1;
-                word &= word - 1u;
-                int j = ((base + lane) << 5) + bit;
-                uint32_t v = (uint32_t)j;
-                if (exo) {
-                    int lo = min(i, j), hi = max(i, j);
-                    if (in_sorted(s.topo.excl_idx, exo[lo], exo[lo + 1], hi)) v |= kExclBit;
-                }
-                if (slot < b.cap_f) rowf[slot] = v;
-                ++slot;
-            }
-            total += __shfl(x, kWave - 1, kWave);
-        }
-        if (lane == 0) {
-            b.nlf_cnt[R1(b, c) + i] = total;
-            atomicMax(&fl[2], total);
-            if (total > b.cap_f) atomicOr(&fl[0], 4);
-        }
-    }
+    rtick(2);
     // HCT sums over every half-row entry, both directions
     for (int i = wave; i < n && grid_ok; i += kMcWaves) {
         const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
@@ -330,10 +526,9 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
         size_t o = R1(b, c) + i;
         b.a[o] = 1.0 / fmax(1.0 / s.rstar[i] - 0.5 * sum[i], 2.0);
         for (int d = 0; d < 4; ++d) ref[4 * i + d] = pos[4 * i + d];
-        dhalf[i] = 0.0;
-        flag[i] = 0;
     }
     __syncthreads();
+    rtick(4);
     // SASA fold of every atom over its SASA row (the incremental SASA change
     // of a step is measured against these).  The fresh total energy the
     // reference assigns to curE here (total_e, src/physics_engine.cpp:2509)
@@ -347,6 +542,13 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
         int clamped;
         double sa = sasa_fold_wave(pi, pos, s.prm, ri, b.nls + (R1(b, c) + i) * (size_t)b.cap_s, ms, &clamped);
         if (lane == 0) b.part[5 * (R1(b, c) + i) + 4] = kGammaSA * fmax(0.0, sa);
+    }
+    // the stage aliases the step's LDS arrays: clear them for the next step
+    __syncthreads();
+    rtick(5);
+    for (int i = tid; i < n; i += kMcBlock) {
+        dhalf[i] = 0.0;
+        flag[i] = 0;
     }
     if (tid == 0) {
         ChainState* cs = b.st + c;
@@ -502,6 +704,13 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     const bool lds_atoms = n <= kLdsAtoms;
     double* dhalf = lds_atoms ? dsm : b.dhalf + R1(b, c);
     int* flag = lds_atoms ? (int*)(dsm + n) : b.flag + R1(b, c);
+    // list-build stage (aliases dhalf / flag when in LDS: they are idle
+    // during a build and cleared after it)
+    const bool lds_stage = n <= kLdsRebuild;
+    float4* stage_cp = lds_stage ? (float4*)dsm : b.stage + R1(b, c);
+    int* stage_ci = lds_stage ? (int*)((float4*)dsm + n) : b.stage_idx + R1(b, c);
+    uint32_t* stage_rb = lds_stage ? (uint32_t*)(stage_ci + n)
+                                   : b.bm + (size_t)c * 2 * kMcWaves * (size_t)((n + 31) / 32);
     const uint32_t* nlf = b.nlf + R1(b, c) * (size_t)b.cap_f;
     int* aff = b.aff + R1(b, c);
     const int* nlf_cnt = b.nlf_cnt + R1(b, c);
@@ -544,7 +753,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     for (;;) {
         if (pending) {
             if (tid == 0) t_prev = wall_clock64();
-            chain_rebuild(b, s, c, pending - 1, dhalf, flag);
+            chain_rebuild(b, s, c, pending - 1, dhalf, flag, stage_cp, stage_ci, stage_rb, lds_stage);
             tick(7);
             if (tid == 0) pending = 0;
         }
@@ -1240,11 +1449,16 @@ hipError_t launch_chain_energy(ChainBatch& b, const DevSys& s, hipStream_t st) {
 
 hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int L, hipStream_t st) {
     if (L <= 0) return hipSuccess;
-    size_t lds = b.n <= kLdsAtoms ? (size_t)b.n * (sizeof(double) + sizeof(int)) : 0;
+    auto stage_bytes = [](int n) {
+        return (size_t)n * (sizeof(float4) + sizeof(int)) + (size_t)2 * kMcWaves * ((n + 31) / 32) * sizeof(uint32_t);
+    };
+    size_t lds = b.n <= kLdsRebuild ? stage_bytes(b.n)
+                 : b.n <= kLdsAtoms ? (size_t)b.n * (sizeof(double) + sizeof(int))
+                                    : 0;
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_mc, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(kLdsAtoms * (sizeof(double) + sizeof(int))));
+        size_t cap = std::max(stage_bytes(kLdsRebuild), (size_t)kLdsAtoms * (sizeof(double) + sizeof(int)));
+        hipError_t e = hipFuncSetAttribute((const void*)k_mc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
