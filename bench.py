@@ -35,8 +35,10 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X vector FP32
 # algorithmic FP32 operation census per unit of work (counted from the
 # reference formulas): pair energy 40, HCT change of a cross pair 2 x 22,
 # SASA fold entry 12, Rodrigues rotation 38 per moved atom, boundary
-# dihedral record old + new 2 x 47
-CENSUS = {"pair_evals": 40, "cross_pairs": 44, "sasa_entries": 12, "moved_atoms": 38, "dih_records": 94}
+# dihedral record old + new 2 x 47; list builds (inside the MC kernel): the
+# Born sum of a half-row pair 55, a SASA-row entry folded 12
+CENSUS = {"pair_evals": 40, "cross_pairs": 44, "sasa_entries": 12, "moved_atoms": 38, "dih_records": 94,
+          "build_pairs": 55, "build_sasa": 12}
 
 
 def log(*a):
@@ -55,12 +57,23 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--force-reps", type=int, default=200)
+    ap.add_argument("--mala-steps", type=int, default=2000)
+    ap.add_argument("--pt-proteins", type=int, default=16)
+    ap.add_argument("--pt-steps", type=int, default=100)
+    ap.add_argument("--no-pt", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 256 MiB device-to-device copies (the runtime's copy kernel), "
+                         "the reference for the FETCH_SIZE / WRITE_SIZE counter scale")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the path has no multi-GPU form in this build "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        _copy_calibration()
+        return
 
     import numpy as np
     import protein_physics as PP
@@ -91,6 +104,7 @@ def main():
     kernel_ms = 0.0
     launches = 0
     rebuild_ms = 0.0
+    rebuilds = 0
     acc = 0
     t_start = time.perf_counter()
     for k in range(args.steps):
@@ -98,7 +112,8 @@ def main():
         tm = ctx.mc_timing()
         kernel_ms += tm["mc_kernel_ms"]
         launches += int(tm["mc_launches"])
-        rebuild_ms += tm["rebuild_ms"]
+        rebuild_ms += tm["rebuild_chain_ms"]
+        rebuilds += int(tm["rebuilds"])
         for key in CENSUS:
             work[key] += tm[key]
         acc += sum(x["accepted"] for x in r["stats"])
@@ -108,26 +123,44 @@ def main():
     value = evals / elapsed
 
     # single chain, same mode (latency-bound)
-    t1 = time.perf_counter()
-    ctx.mc_run(N.MC_TORSION30, args.single_steps, n_chains=1, temperature=kt, seed=77)
-    single = args.single_steps / (time.perf_counter() - t1)
-    log("single chain: %.1f evals/s" % single)
+    single = None
+    if args.single_steps > 0:
+        t1 = time.perf_counter()
+        ctx.mc_run(N.MC_TORSION30, args.single_steps, n_chains=1, temperature=kt, seed=77)
+        single = args.single_steps / (time.perf_counter() - t1)
+        log("single chain: %.1f evals/s" % single)
 
     # full evaluation (calculate_potential: list + Born + all terms) against
     # the same evaluation plus the backward kernels (forces and torsion
     # gradients), same coordinates, host round trip included in both
-    forces = _forces_rate(ctx, args.force_reps)
-    log("energy %.1f evals/s, forces %.1f evals/s" % (forces["energy_evals_per_s"], forces["force_evals_per_s"]))
+    forces = None
+    if args.force_reps > 0:
+        forces = _forces_rate(ctx, args.force_reps)
+        log("energy %.1f evals/s, forces %.1f evals/s" % (forces["energy_evals_per_s"], forces["force_evals_per_s"]))
 
     flops = sum(CENSUS[k] * work[k] for k in CENSUS)
     achieved = flops / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else 0.0
     roofline = {"bound": "mfma", "roof": "FP32 vector ALU peak (no MFMA: the pair sums are not a dense contraction)", "achieved": achieved,
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                "traffic": None,
+                "traffic": _traffic(args, launches),
                 "kernel": "k_mc", "kernel_ms_per_launch": kernel_ms / max(launches, 1),
                 "algorithmic_flop_per_launch": flops / max(launches, 1),
                 "census": {k: work[k] / max(launches, 1) for k in CENSUS},
-                "rebuild_ms_total": rebuild_ms, "kernel_ms_total": kernel_ms}
+                "kernel_ms_total": kernel_ms,
+                "list_builds": rebuilds, "list_build_chain_ms": rebuild_ms,
+                "note": "list builds run inside k_mc; their chain-time is summed over chains"}
+
+    mala = None
+    if args.mala_steps > 0:
+        mala = _mala_rate(ctx, args, kt)
+        log("MALA: %.1f steps/s" % mala["steps_per_s"])
+    pt = None
+    if not args.no_pt:
+        pt = _pt_compare(s, args, PP)
+        log("PT: batched MALA %.0f accepted/s, plain per-replica MC %.0f accepted/s"
+            % (pt["batched_mala_pt"]["accepted_per_s"], pt["plain_per_replica_mc"]["accepted_per_s"]))
+    # the energy context is reused by the later calls: reload the bench system
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], csr)
 
     cpu = None
     if not args.no_cpu:
@@ -153,10 +186,45 @@ def main():
         "single_chain_evals_per_s": single,
         "accept_rate": acc / max(evals, 1),
         "forces": forces,
+        "mala": mala,
+        "parallel_tempering": pt,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _traffic(args, launches):
+    """HBM bytes per k_mc launch of this workload from the committed counter
+    measurement (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
+    calibration), when the workload matches it; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "k_mc_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if (w.get("residues"), w.get("chains"), w.get("mc_steps")) != (args.residues, args.chains, args.mc_steps):
+        return None
+    return t.get("traffic_bytes_per_launch")
+
+
+def _copy_calibration():
+    """Three 256 MiB device-to-device copies through the HIP runtime (its
+    copy kernel reads and writes exactly that many bytes)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = 256 << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("copied 3 x %d bytes" % n)
 
 
 def _forces_rate(ctx, reps):
@@ -173,6 +241,90 @@ def _forces_rate(ctx, reps):
     return {"energy_evals_per_s": 1.0 / te, "force_evals_per_s": 1.0 / tf, "force_over_energy_cost": tf / te,
             "note": "full calculate_potential evaluation vs the same plus dE/dx (Born chain rule) and "
                     "dE/dtheta, one system, host round trip included"}
+
+
+def _mala_rate(ctx, args, kt):
+    """Gradient-guided torsion steps on the same system: every step a full
+    calculate_potential evaluation and gradient of the proposal (fresh list),
+    h = 0.02 rad, as one captured HIP graph per step."""
+    from alma import _native as N
+    ctx.mc_run(N.MC_MALA, 20, n_chains=1, temperature=kt, mala_step=0.02, seed=11)
+    t0 = time.perf_counter()
+    r = ctx.mc_run(N.MC_MALA, args.mala_steps, n_chains=1, temperature=kt, mala_step=0.02, seed=12)
+    dt = time.perf_counter() - t0
+    st = r["stats"][0]
+    return {"steps": args.mala_steps, "steps_per_s": args.mala_steps / dt,
+            "accept_rate": st["accepted"] / max(st["moves"], 1), "mala_step_rad": 0.02,
+            "note": "one system; each step = proposal + fresh list + energy + gradient (force evaluation)"}
+
+
+def _segment(s, n_res):
+    """The first n_res residues of a chain, as its own protein."""
+    import numpy as np
+    first = int(np.min(s["res_idx"]))
+    keep = np.flatnonzero(np.asarray(s["res_idx"]) < first + n_res)
+    return {"xyz": s["xyz"][keep], "charge": s["charge"][keep], "rstar": s["rstar"][keep],
+            "eps": s["eps"][keep], "resnames": [s["resnames"][k] for k in keep],
+            "atomnames": [s["atomnames"][k] for k in keep], "res_idx": np.asarray(s["res_idx"])[keep]}
+
+
+def _pt_compare(s, args, PP):
+    """Batched parallel tempering with MALA moves (all proteins and slots in
+    one sequence of launches) against plain per-replica device MC (one launch
+    per protein, its replicas as chains at their slot temperatures), in
+    accepted moves per second.  Proteins: the first L residues of the bench
+    chain, L ~ LogNormal(ln 180, 0.6) truncated to [40, 300]; 8 slots,
+    T_k = 280 (450/280)^(k/7) K."""
+    import numpy as np
+    from alma import _native as N
+    rng = np.random.default_rng(7)
+    sizes = []
+    while len(sizes) < args.pt_proteins:
+        v = int(round(rng.lognormal(np.log(180.0), 0.6)))
+        if 40 <= v <= args.residues:
+            sizes.append(v)
+    prots, csrs = [], []
+    for L in sizes:
+        p = _segment(s, L)
+        t = PP.BondTopology()
+        t.build(p["resnames"], p["atomnames"], [int(v) for v in p["res_idx"]])
+        prots.append(p)
+        csrs.append(_csr(t))
+    temps = [KB * 280.0 * (450.0 / 280.0) ** (k / 7.0) for k in range(8)]
+    R = len(temps)
+    ctx = N.Context(0)
+    ctx.pt_set(prots, csrs, R, temps, [p["xyz"] for p in prots for _ in range(R)])
+    ctx.pt_run(5, exchange_every=10, mala_step=0.02, seed=3)
+    t0 = time.perf_counter()
+    r = ctx.pt_run(args.pt_steps, exchange_every=10, mala_step=0.02, seed=3)
+    dt = time.perf_counter() - t0
+    tm = ctx.pt_timing()
+    acc_pt = sum(x["accepted"] for x in r["stats"])
+    moves_pt = sum(x["moves"] for x in r["stats"])
+    swaps = r["swaps"].sum(axis=0)
+    # plain device MC: per protein one launch, R chains at the slot temperatures
+    plain_steps = max(10, args.pt_steps)
+    acc_pl = 0
+    moves_pl = 0
+    t_pl = 0.0
+    for p, c in zip(prots, csrs):
+        ctx.set_system(p["xyz"], p["charge"], p["rstar"], p["eps"], c)
+        t1 = time.perf_counter()
+        rr = ctx.mc_run(N.MC_ENSEMBLE, plain_steps, n_chains=R, seed=5, chain_temperatures=temps)
+        t_pl += time.perf_counter() - t1
+        acc_pl += sum(x["accepted"] for x in rr["stats"])
+        moves_pl += sum(x["moves"] for x in rr["stats"])
+    ctx.close()
+    return {
+        "proteins": len(prots), "slots": R, "residues": sizes,
+        "atoms_total": int(tm["atoms"]),
+        "batched_mala_pt": {"steps": args.pt_steps, "accepted_per_s": acc_pt / dt, "accept_rate": acc_pt / max(moves_pt, 1),
+                            "exchange_every": 10, "swaps_attempted": int(swaps[0]), "swaps_accepted": int(swaps[1]),
+                            "device_ms": tm["device_ms"], "pair_kernel_lanes": int(tm["pair_lanes"])},
+        "plain_per_replica_mc": {"steps": plain_steps, "accepted_per_s": acc_pl / t_pl,
+                                 "accept_rate": acc_pl / max(moves_pl, 1), "mode": "generate_ensemble moves"},
+        "note": "residue counts capped at the bench chain's 300 (the protein set is cut from it)",
+    }
 
 
 def _csr(t):

@@ -100,6 +100,8 @@ typedef struct {
     double mala_step;        /* MALA torsion step (rad)                                 */
     uint32_t seed;
     int32_t trace;           /* record per-step trace (chain 0) when nonzero            */
+    const double* chain_temperature;  /* optional kT per chain (replicas of a ladder);
+                                         NULL: every chain at `temperature`            */
 } alma_mc_params;
 
 typedef struct {
@@ -123,11 +125,13 @@ alma_status alma_mc_run(alma_ctx* ctx, const alma_mc_params* p, double* xyz_out,
 alma_status alma_mc_trace(alma_ctx* ctx, int32_t* kind, int32_t* bond, double* delta,
                           double* dE, int32_t* accepted, double* dE_parts);
 
-/* Measurement of the last alma_mc_run (no reference counterpart): out[9] =
+/* Measurement of the last alma_mc_run (no reference counterpart): out[11] =
  * {device MC kernel ms (HIP events on the context stream), MC launches,
- *  list-rebuild ms, rebuild batches, cross pairs, in-cutoff pair
+ *  chain-time spent in list builds (ms, summed over chains; builds run
+ *  inside the MC kernel), list rebuilds, cross pairs, in-cutoff pair
  *  evaluations, SASA row entries folded, moved atoms, boundary dihedral
- *  records}, summed over chains. */
+ *  records, list-build half-row pairs, list-build SASA-row entries},
+ *  summed over chains. */
 alma_status alma_mc_timing(alma_ctx* ctx, double* out);
 
 /* ---- forces (no reference counterpart) ---- */

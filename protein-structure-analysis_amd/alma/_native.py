@@ -38,7 +38,7 @@ class McParams(C.Structure):
     _fields_ = [("mode", C.c_int32), ("n_chains", C.c_int32), ("steps", C.c_int32),
                 ("snapshot_every", C.c_int32), ("temperature", C.c_double),
                 ("max_angle", C.c_double), ("mala_step", C.c_double), ("seed", C.c_uint32),
-                ("trace", C.c_int32)]
+                ("trace", C.c_int32), ("chain_temperature", _D)]
 
 
 class McStats(C.Structure):
@@ -217,9 +217,14 @@ class Context:
         return a
 
     def mc_run(self, mode, steps, n_chains=1, temperature=0.6, max_angle=0.12, seed=5489,
-               snapshot_every=0, mala_step=0.02, trace=False):
+               snapshot_every=0, mala_step=0.02, trace=False, chain_temperatures=None):
+        ct = None
+        if chain_temperatures is not None:
+            ct = np.ascontiguousarray(chain_temperatures, np.float64)
+            if len(ct) != n_chains:
+                raise ValueError("one temperature per chain")
         p = McParams(mode, n_chains, steps, snapshot_every, temperature, max_angle, mala_step,
-                     seed, 1 if trace else 0)
+                     seed, 1 if trace else 0, _p(ct, C.c_double) if ct is not None else None)
         xyz = np.zeros((n_chains, self.n, 3))
         st = (McStats * n_chains)()
         nsnap = steps // snapshot_every if snapshot_every else 0
@@ -245,10 +250,10 @@ class Context:
         return out
 
     def mc_timing(self) -> dict:
-        out = np.zeros(9)
+        out = np.zeros(11)
         self._check(lib().alma_mc_timing(self._h, _p(out, C.c_double)))
-        keys = ("mc_kernel_ms", "mc_launches", "rebuild_ms", "rebuild_batches", "cross_pairs",
-                "pair_evals", "sasa_entries", "moved_atoms", "dih_records")
+        keys = ("mc_kernel_ms", "mc_launches", "rebuild_chain_ms", "rebuilds", "cross_pairs",
+                "pair_evals", "sasa_entries", "moved_atoms", "dih_records", "build_pairs", "build_sasa")
         return dict(zip(keys, out.tolist()))
 
     def forces(self):

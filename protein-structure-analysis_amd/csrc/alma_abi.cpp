@@ -482,7 +482,8 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
         ChainState& x = st[c];
         x = ChainState{};
         x.cur_max = p.max_angle;
-        x.temperature = p.temperature;
+        x.temperature = p.chain_temperature ? p.chain_temperature[c] : p.temperature;
+        if (!(x.temperature > 0.0)) throw Fail{ALMA_EINVAL, "temperature must be positive"};
         x.mala_step = p.mala_step;
         x.wavail = b.wcap;
         x.mode = p.mode;
@@ -569,7 +570,7 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     HIP_TRY(launch_chain_energy(b, s, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
-    for (int k = 0; k < 5; ++k) ctx->work[k] = 0;
+    for (int k = 0; k < 7; ++k) ctx->work[k] = 0;
     // list rebuilds run inside the MC kernel: chain-time spent in them
     // (wall clock of each chain's workgroup) and their count
     for (int c = 0; c < nch; ++c) {
@@ -582,6 +583,8 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
         ctx->work[2] += st[c].n_sasa;
         ctx->work[3] += st[c].n_moved;
         ctx->work[4] += st[c].n_dih;
+        ctx->work[5] += st[c].n_build_pairs;
+        ctx->work[6] += st[c].n_build_sasa;
     }
     if (std::getenv("ALMA_MC_PROFILE")) {
         ChainState& x = st[0];
@@ -1330,7 +1333,7 @@ alma_status alma_mc_timing(alma_ctx* ctx, double* out) {
         out[1] = (double)ctx->n_mc_launch;
         out[2] = ctx->t_rebuild_ms;
         out[3] = (double)ctx->n_rebuild_launch;
-        for (int k = 0; k < 5; ++k) out[4 + k] = (double)ctx->work[k];
+        for (int k = 0; k < 7; ++k) out[4 + k] = (double)ctx->work[k];
     });
 }
 

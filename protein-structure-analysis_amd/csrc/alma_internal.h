@@ -124,6 +124,7 @@ struct ChainState {
     // evaluations (old + new), SASA row entries folded, moved atoms,
     // boundary dihedral records
     long long n_cross, n_pair_eval, n_sasa, n_moved, n_dih;
+    long long n_build_pairs, n_build_sasa;   // list builds: half-row pairs (Born), SASA-row entries folded
 };
 
 // Batch of chains of one system (same topology and parameters, own state).
@@ -312,7 +313,7 @@ struct alma_ctx {
     // HIP-event timing of the last MC run: device MC kernel, list rebuilds
     double t_mc_ms = 0.0, t_rebuild_ms = 0.0;
     long long n_mc_launch = 0, n_rebuild_launch = 0;
-    long long work[5] = {0, 0, 0, 0, 0};
+    long long work[7] = {0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // host copy of the loaded topology (the MALA path builds a batch from it)
     HostTopo topo_copy;

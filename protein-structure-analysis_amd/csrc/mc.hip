@@ -135,7 +135,7 @@ template <bool kLds>
 __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, int c, const Grid& gr,
                                           const int* cell_of, const int* start, const float4* cp, const int* ci,
                                           uint32_t* rb, int* pref, int* ex, int& max_h, int& max_s, int& max_f,
-                                          int& ovf) {
+                                          int& ovf, long long& sum_h, long long& sum_s) {
     const int n = b.n;
     const int lane = lane_id(), wave = threadIdx.x / kWave;
     const int W = (n + 31) / 32;
@@ -323,6 +323,8 @@ __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, 
             b.nlf_cnt[R1(b, c) + i] = total;
             max_h = max(max_h, nh);
             max_s = max(max_s, ns);
+            sum_h += nh;
+            sum_s += ns;
             max_f = max(max_f, total);
             if (nh > b.cap_h || ns > b.cap_s) ovf |= 1;
             if (total > b.cap_f) ovf |= 4;
@@ -486,15 +488,19 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
     rtick(1);
     const bool topo = s.has_topo;
     int max_h = 0, max_s = 0, max_f = 0, ovf = 0;
+    long long sum_h = 0, sum_s = 0;
     if (grid_ok) {
         if (rbits_lds && lds_cells)
-            fill_rows<true>(b, s, c, gr, cell_of, sstart, cp, ci, rb, pref, ex, max_h, max_s, max_f, ovf);
+            fill_rows<true>(b, s, c, gr, cell_of, sstart, cp, ci, rb, pref, ex, max_h, max_s, max_f, ovf, sum_h, sum_s);
         else
-            fill_rows<false>(b, s, c, gr, cell_of, start, cp, ci, rb, pref, ex, max_h, max_s, max_f, ovf);
+            fill_rows<false>(b, s, c, gr, cell_of, start, cp, ci, rb, pref, ex, max_h, max_s, max_f, ovf, sum_h,
+                             sum_s);
     }
     (void)topo;
     if (lane == 0) {
         atomicMax(&fl[1], max_h);
+        atomicAdd((unsigned long long*)&b.st[c].n_build_pairs, (unsigned long long)sum_h);
+        atomicAdd((unsigned long long*)&b.st[c].n_build_sasa, (unsigned long long)sum_s);
         atomicMax(&fl[2], max_f);
         atomicMax(&fl[3], max_s);
         if (ovf) atomicOr(&fl[0], ovf);
