@@ -668,6 +668,11 @@ void free_batch(alma_ctx* ctx) {
     dev_free(b.flags);
     dev_free(b.excl_off);
     dev_free(b.excl_idx);
+    dev_free(b.exs_off);
+    dev_free(b.exs_idx);
+    dev_free(b.nlf);
+    dev_free(b.nlf_cnt);
+    dev_free(b.stage);
     dev_free(b.rb_i);
     dev_free(b.rb_j);
     dev_free(b.run_off);
@@ -706,14 +711,15 @@ void up(T** p, const std::vector<T>& v) {
 
 // initial evaluation (energy and gradient of the loaded states); returns the
 // longest neighbour row seen
-int batch_init_eval(alma_ctx* ctx) {
+int batch_init_eval(alma_ctx* ctx, int* max_full = nullptr) {
     Batch& b = ctx->batch;
-    HIP_TRY(hipMemsetAsync(b.flags, 0, sizeof(int), ctx->stream));
+    HIP_TRY(hipMemsetAsync(b.flags, 0, 2 * sizeof(int), ctx->stream));
     HIP_TRY(launch_batch_step(b, 1, ctx->stream));
-    int maxrow = 0;
-    HIP_TRY(hipMemcpyAsync(&maxrow, b.flags, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    int rows[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(rows, b.flags, sizeof(rows), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    return maxrow;
+    if (max_full) *max_full = rows[1];
+    return rows[0];
 }
 
 void batch_states(alma_ctx* ctx, std::vector<BState>& st) {
@@ -787,7 +793,8 @@ void pt_set(alma_ctx* ctx, const alma_pt_systems* ps, const double* xyz) {
     b.n_atoms = (int)total;
     // topology, packed with local indices
     std::vector<BProt> prot(P);
-    std::vector<int> excl_off, excl_idx, rb_i, rb_j, run_off, dih_a, dih_b, dih_c, dih_d, dih_toff, dih_n, ss_i, ss_j;
+    std::vector<int> excl_off, excl_idx, exs_off, exs_idx, rb_i, rb_j, run_off, dih_a, dih_b, dih_c, dih_d, dih_toff,
+        dih_n, ss_i, ss_j;
     std::vector<int2> runs;
     std::vector<double> dih_v2, dih_gamma;
     for (int p = 0; p < P; ++p) {
@@ -799,6 +806,22 @@ void pt_set(alma_ctx* ctx, const alma_pt_systems* ps, const double* xyz) {
         const int eb = (int)excl_idx.size();
         for (int i = 0; i <= n; ++i) excl_off.push_back(eb + t->excl_off[i]);
         excl_idx.insert(excl_idx.end(), t->excl_idx, t->excl_idx + t->excl_off[n]);
+        {
+            // both directions of every exclusion, ascending per atom
+            std::vector<std::vector<int>> rows(n);
+            for (int i = 0; i < n; ++i)
+                for (int k = t->excl_off[i]; k < t->excl_off[i + 1]; ++k) {
+                    rows[i].push_back(t->excl_idx[k]);
+                    rows[t->excl_idx[k]].push_back(i);
+                }
+            exs_off.push_back((int)exs_idx.size());
+            for (int i = 0; i < n; ++i) {
+                std::sort(rows[i].begin(), rows[i].end());
+                rows[i].erase(std::unique(rows[i].begin(), rows[i].end()), rows[i].end());
+                exs_idx.insert(exs_idx.end(), rows[i].begin(), rows[i].end());
+                exs_off.push_back((int)exs_idx.size());
+            }
+        }
         pr.rot_off = (int)rb_i.size();
         pr.n_rot = t->n_rot;
         rb_i.insert(rb_i.end(), t->rb_i, t->rb_i + t->n_rot);
@@ -895,12 +918,15 @@ void pt_set(alma_ctx* ctx, const alma_pt_systems* ps, const double* xyz) {
     dev_alloc(&b.cell_of, total);
     dev_alloc(&b.cell_tmp, total);
     dev_alloc(&b.cell_atoms, total);
+    dev_alloc(&b.stage, total);
     dev_alloc(&b.cell_cnt, coff);
     dev_alloc(&b.cell_start, coff);
     dev_alloc(&b.nl_cnt, total);
     dev_alloc(&b.flags, 4);
     up(&b.excl_off, excl_off);
     up(&b.excl_idx, excl_idx);
+    up(&b.exs_off, exs_off);
+    up(&b.exs_idx, exs_idx);
     up(&b.rb_i, rb_i);
     up(&b.rb_j, rb_j);
     up(&b.run_off, run_off);
@@ -927,10 +953,16 @@ void pt_set(alma_ctx* ctx, const alma_pt_systems* ps, const double* xyz) {
     ctx->h_bprot = prot;
     // size the rows from a first evaluation, then evaluate for real
     b.cap = 256;
+    b.cap_f = 512;
     dev_alloc(&b.nl, (size_t)total * b.cap);
-    int maxrow = batch_init_eval(ctx);
+    dev_alloc(&b.nlf, (size_t)total * b.cap_f);
+    dev_alloc(&b.nlf_cnt, total);
+    int maxfull = 0;
+    int maxrow = batch_init_eval(ctx, &maxfull);
     b.cap = (maxrow * 3 / 2 + 64 + 63) / 64 * 64;
+    b.cap_f = (maxfull * 3 / 2 + 64 + 63) / 64 * 64;
     dev_alloc(&b.nl, (size_t)total * b.cap);
+    dev_alloc(&b.nlf, (size_t)total * b.cap_f);
     up(&b.st, st);   // clear the overflow status of the sizing pass
     batch_init_eval(ctx);
     batch_autotune(ctx);

@@ -106,6 +106,7 @@ enum { CS_RUN = 0, CS_REBUILD = 1, CS_TARGET = 2, CS_WORDS = 3, CS_ERROR = 4, CS
 struct ChainState {
     double cur_e;        // running energy (curE): fresh total at the last list build + de_since
     double de_since;     // sum of accepted dE since the last list build
+    int rb_pending;      // the last accepted move drifted an atom past 1 A: build at the next step
     double cur_max;      // adaptive step
     double temperature;  // kT, kcal/mol
     double mala_step;
@@ -176,7 +177,7 @@ struct ChainBatch {
 // ---- batched systems: replicas of proteins of uneven size (batch.hip) ----
 struct BProt {
     int n;          // atoms
-    int excl_base;  // first of this protein's n + 1 entries in excl_off
+    int excl_base;  // first of this protein's n + 1 entries in excl_off (and exs_off)
     int rot_off;    // first rotatable bond (run_off index)
     int n_rot;
     int dih_off;    // first dihedral record (dih_a..d, dih_toff index)
@@ -233,12 +234,18 @@ struct Batch {
     int* cell_atoms = nullptr;
     int* cell_cnt = nullptr;
     int* cell_start = nullptr;
-    uint32_t* nl = nullptr;    // n_atoms * cap, local j | excl bit
+    uint32_t* nl = nullptr;    // n_atoms * cap, half rows (j > i), local j | excl bit, scan order
     int* nl_cnt = nullptr;
+    float4* stage = nullptr;   // n_atoms: cell-ordered FP32 coordinates relative to each grid origin
+    uint32_t* nlf = nullptr;   // n_atoms * cap_f, full rows (both directions), scan order
+    int* nlf_cnt = nullptr;
+    int cap_f = 0;
     int* flags = nullptr;      // [0] largest row
     // topology, local atom indices, packed over proteins
     int* excl_off = nullptr;
     int* excl_idx = nullptr;
+    int* exs_off = nullptr;    // symmetric exclusions (both directions, ascending), same offsets layout
+    int* exs_idx = nullptr;
     int* rb_i = nullptr;
     int* rb_j = nullptr;
     int* run_off = nullptr;

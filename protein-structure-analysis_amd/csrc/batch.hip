@@ -308,8 +308,29 @@ __global__ void kb_rank(Batch b) {
     b.cell_atoms[sy.aoff + lo + r] = i;
 }
 
-// one wavefront per atom: 27 cells dz -> dy -> dx, members ascending, j > i,
-// d^2 < 196; ballot prefix keeps the scan order
+// cell-ordered FP32 coordinates relative to the system's grid origin: the
+// candidate scan below reads them contiguously instead of gathering FP64
+// positions
+__global__ void kb_stage(Batch b) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= b.n_atoms) return;
+    const int s = b.atom_sys[g];
+    const BSys& sy = b.sys[s];
+    const Grid& gr = b.grid[s];
+    const int j = b.cell_atoms[g];   // slot g - aoff of the system's cell order
+    const double* x = b.posn + 4 * ((size_t)sy.aoff + j);
+    b.stage[g] = make_float4((float)(x[0] - gr.lo[0]), (float)(x[1] - gr.lo[1]), (float)(x[2] - gr.lo[2]), 0.0f);
+}
+
+// one wavefront per atom: 27 cells dz -> dy -> dx, members ascending, d^2 <
+// 196 (the reference's test, FP64 where FP32 cannot decide); partners j > i
+// form the half row (the reference's list, whose order the SASA fold
+// follows), all partners the full row (both directions, so the sums and
+// gradients below need no atomics); ballot prefixes keep the scan order.
+// The 27 cells are concatenated into one candidate range (lane c < 27 holds
+// cell c's bounds) walked 64 at a time with the next chunk's loads in flight;
+// exclusions within 64 indices of i are a 128-bit register mask, others a
+// search of the sorted exclusion row.
 __global__ void __launch_bounds__(kBlock) kb_nl(Batch b) {
     int g = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
     if (g >= b.n_atoms) return;
@@ -318,7 +339,10 @@ __global__ void __launch_bounds__(kBlock) kb_nl(Batch b) {
     const Grid& gr = b.grid[s];
     const int lane = lane_id();
     if (gr.ncell > sy.max_cells) {   // grid too large: empty rows, the step is flagged
-        if (lane == 0) b.nl_cnt[g] = 0;
+        if (lane == 0) {
+            b.nl_cnt[g] = 0;
+            b.nlf_cnt[g] = 0;
+        }
         return;
     }
     const BProt& pr = b.prot[sy.prot];
@@ -327,46 +351,139 @@ __global__ void __launch_bounds__(kBlock) kb_nl(Batch b) {
     const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
     const int* start = b.cell_start + sy.coff;
     const int* atoms = b.cell_atoms + sy.aoff;
-    int c = b.cell_of[g];
-    int cx = c % gr.nc[0], cy = (c / gr.nc[0]) % gr.nc[1], cz = c / (gr.nc[0] * gr.nc[1]);
-    int eb = b.excl_off[pr.excl_base + i], ee = b.excl_off[pr.excl_base + i + 1];
-    uint32_t* row = b.nl + (size_t)g * b.cap;
-    int count = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        int z = cz + dz;
-        if (z < 0 || z >= gr.nc[2]) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            int y = cy + dy;
-            if (y < 0 || y >= gr.nc[1]) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                int xx = cx + dx;
-                if (xx < 0 || xx >= gr.nc[0]) continue;
-                int cc = xx + gr.nc[0] * (y + gr.nc[1] * z);
-                int lo = start[cc], hi = start[cc + 1];
-                for (int base = lo; base < hi; base += kWave) {
-                    int k = base + lane;
-                    bool keep = false;
-                    int j = 0;
-                    if (k < hi) {
-                        j = atoms[k];
-                        if (j > i) keep = dist2_exact(pi, x + 4 * j) < kNlRcut2;
-                    }
-                    uint64_t m = __ballot(keep);
-                    if (keep) {
-                        int slot = count + __popcll(m & ((1ull << lane) - 1ull));
-                        uint32_t v = (uint32_t)j;
-                        if (in_sorted(b.excl_idx, eb, ee, j)) v |= kExclBit;
-                        if (slot < b.cap) row[slot] = v;
-                    }
-                    count += __popcll(m);
-                }
+    const float4* stage = b.stage + sy.aoff;
+    const int c = b.cell_of[g];
+    const int cx = c % gr.nc[0], cy = (c / gr.nc[0]) % gr.nc[1], cz = c / (gr.nc[0] * gr.nc[1]);
+    const float pfx = (float)(pi[0] - gr.lo[0]), pfy = (float)(pi[1] - gr.lo[1]), pfz = (float)(pi[2] - gr.lo[2]);
+
+    // lane c < 27: cell c of the scan (dz slowest, dx fastest); cells outside
+    // the grid or whose box lies beyond the radius (FP32 margin 0.1 A^2) are empty
+    int lo_c = 0, n_c = 0;
+    if (lane < 27) {
+        const int dz = lane / 9 - 1, dy = (lane / 3) % 3 - 1, dx = lane % 3 - 1;
+        const int zz = cz + dz, yy = cy + dy, xx = cx + dx;
+        if (zz >= 0 && zz < gr.nc[2] && yy >= 0 && yy < gr.nc[1] && xx >= 0 && xx < gr.nc[0]) {
+            const float fcs = (float)gr.cs;
+            const float ox = pfx - cx * fcs, oy = pfy - cy * fcs, oz = pfz - cz * fcs;
+            const float gx = dx < 0 ? ox : dx > 0 ? fcs - ox : 0.0f;
+            const float gy = dy < 0 ? oy : dy > 0 ? fcs - oy : 0.0f;
+            const float gz = dz < 0 ? oz : dz > 0 ? fcs - oz : 0.0f;
+            if (gx * gx + gy * gy + gz * gz <= (float)kNlRcut2 + 0.1f) {
+                const int cc = xx + gr.nc[0] * (yy + gr.nc[1] * zz);
+                lo_c = start[cc];
+                n_c = start[cc + 1] - lo_c;
             }
         }
     }
+    int off_c = n_c;   // inclusive scan over lanes 0..31
+    for (int o = 1; o < 32; o <<= 1) {
+        const int t = __shfl_up(off_c, o, kWave);
+        if (lane >= o) off_c += t;
+    }
+    const int total = __shfl(off_c, 26, kWave);
+    off_c -= n_c;                    // first candidate of cell c
+    const int d_c = lo_c - off_c;    // candidate v of cell c is atoms[v + d_c]
+    int offs[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) offs[q] = __builtin_amdgcn_readlane(off_c, q);
+
+    // exclusions of i: offsets -64..63 as bits of (wlo, whi), the rest flagged
+    const int eb = b.exs_off[pr.excl_base + i], ee = b.exs_off[pr.excl_base + i + 1];
+    uint32_t wm[4] = {0u, 0u, 0u, 0u};
+    bool far = false;
+    for (int e0 = eb; e0 < ee; e0 += kWave) {
+        uint32_t m0 = 0u, m1 = 0u, m2 = 0u, m3 = 0u;
+        bool f = false;
+        if (e0 + lane < ee) {
+            const int o = b.exs_idx[e0 + lane] - i + 64;
+            if (o >= 0 && o < 128) {
+                const uint32_t bit = 1u << (o & 31);
+                m0 = (o >> 5) == 0 ? bit : 0u;
+                m1 = (o >> 5) == 1 ? bit : 0u;
+                m2 = (o >> 5) == 2 ? bit : 0u;
+                m3 = (o >> 5) == 3 ? bit : 0u;
+            } else {
+                f = true;
+            }
+        }
+        for (int o = 1; o < kWave; o <<= 1) {
+            m0 |= (uint32_t)__shfl_xor((int)m0, o, kWave);
+            m1 |= (uint32_t)__shfl_xor((int)m1, o, kWave);
+            m2 |= (uint32_t)__shfl_xor((int)m2, o, kWave);
+            m3 |= (uint32_t)__shfl_xor((int)m3, o, kWave);
+        }
+        wm[0] |= m0;
+        wm[1] |= m1;
+        wm[2] |= m2;
+        wm[3] |= m3;
+        far = far || __ballot(f) != 0ull;
+    }
+
+    uint32_t* row = b.nl + (size_t)g * b.cap;
+    uint32_t* rowf = b.nlf + (size_t)g * b.cap_f;
+    int count = 0, countf = 0;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int jn = 0;
+    float4 qn = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto fetch = [&](int base) {
+        // every lane takes part in the shuffle (a lane reading an inactive
+        // lane's register would get no value)
+        const int v = base + lane;
+        int cl = -1;
+#pragma unroll
+        for (int q = 0; q < 27; ++q) cl += v >= offs[q];
+        const int k = v + __shfl(d_c, cl, kWave);
+        if (v < total) {
+            jn = atoms[k];
+            qn = stage[k];
+        }
+    };
+    if (total > 0) fetch(0);
+    for (int base = 0; base < total; base += kWave) {
+        const int j = jn;
+        const float4 q = qn;
+        if (base + kWave < total) fetch(base + kWave);
+        const bool valid = base + lane < total;
+        const float ddx = q.x - pfx, ddy = q.y - pfy, ddz = q.z - pfz;
+        const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        // FP32 staged distance decides every pair except those within
+        // 0.01 A^2 of the radius (|error| < 1e-3 A^2), which take the FP64 test
+        bool keep = valid && j != i && d2 < (float)kNlRcut2 - 0.01f;
+        const bool ambiguous = valid && j != i && !keep && d2 < (float)kNlRcut2 + 0.01f;
+        if (__ballot(ambiguous))
+            if (ambiguous) keep = dist2_exact(pi, x + 4 * j) < kNlRcut2;
+        const bool up = keep && j > i;
+        const uint64_t mf = __ballot(keep), m = __ballot(up);
+        if (keep) {
+            uint32_t v = (uint32_t)j;
+            const int o = j - i + 64;
+            bool ex;
+            if (o >= 0 && o < 128) {
+                const uint32_t w = o < 32 ? wm[0] : o < 64 ? wm[1] : o < 96 ? wm[2] : wm[3];
+                ex = (w >> (o & 31)) & 1u;
+            }
+            else ex = far && in_sorted(b.exs_idx, eb, ee, j);
+            if (ex) v |= kExclBit;
+            int slot = countf + __popcll(mf & below);
+            if (slot < b.cap_f) rowf[slot] = v;
+            if (up) {
+                slot = count + __popcll(m & below);
+                if (slot < b.cap) row[slot] = v;
+            }
+        }
+        count += __popcll(m);
+        countf += __popcll(mf);
+    }
     if (lane == 0) {
         b.nl_cnt[g] = min(count, b.cap);
-        atomicMax(&b.flags[0], count);
-        if (count > b.cap) atomicOr(&b.st[s].flags, 1);
+        b.nlf_cnt[g] = min(countf, b.cap_f);
+        // row-length maxima: same-address atomics serialise in L2, so only
+        // waves above the value already there issue one
+        if (count > __hip_atomic_load(&b.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&b.flags[0], count);
+        if (countf > __hip_atomic_load(&b.flags[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&b.flags[1], countf);
+        if (count > b.cap || countf > b.cap_f) atomicOr(&b.st[s].flags, 1);
     }
 }
 
@@ -379,22 +496,19 @@ __global__ void __launch_bounds__(kBlock) kb_born(Batch b) {
     const int i = g - sy.aoff;
     const double* x = b.posn + 4 * (size_t)sy.aoff;
     const float4* prm = b.prm + sy.aoff;
-    double* sum = b.born_sum + sy.aoff;
     const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
     float ri = prm[i].y;
-    int m = b.nl_cnt[g];
-    const uint32_t* row = b.nl + (size_t)g * b.cap;
+    int m = b.nlf_cnt[g];
+    const uint32_t* row = b.nlf + (size_t)g * b.cap_f;
     double acc = 0.0;
     for (int k = lane; k < m; k += kWave) {
         int j = (int)(row[k] & kIdxMask);
         double r2d = dist2_exact(pi, x + 4 * j);
         float r2 = (float)r2d, r = sqrtf(r2);
-        float rj = prm[j].y;
-        acc += (double)hct_f(r, r2, ri, rj);
-        atomicAdd(&sum[j], (double)hct_f(r, r2, rj, ri));
+        acc += (double)hct_f(r, r2, ri, prm[j].y);
     }
     acc = wave_sum(acc);
-    if (lane == 0) atomicAdd(&sum[i], acc);
+    if (lane == 0) b.born_sum[g] = acc;
 }
 
 __global__ void kb_born_fin(Batch b) {
@@ -406,6 +520,9 @@ __global__ void kb_born_fin(Batch b) {
 }
 
 // ── pair terms and their radial / Born-radius derivatives, L lanes per row ──
+// Over the full rows: every pair is seen from both atoms, so each lane group
+// accumulates its own atom's gradient and dE/da without atomics; the energy
+// of a pair is counted from its lower atom only.
 template <int L>
 __global__ void kb_pair(Batch b) {
     const int groups_per_block = blockDim.x / L;
@@ -419,13 +536,11 @@ __global__ void kb_pair(Batch b) {
     const float4* prm = b.prm + sy.aoff;
     const double* Rs = b.rstar + sy.aoff;
     const double* Es = b.eps64 + sy.aoff;
-    double* G = b.Gn + 3 * (size_t)sy.aoff;
-    double* dEda = b.dEda + sy.aoff;
     const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
     const float4 qi = prm[i];
     const double Ri = Rs[i], Ei = Es[i];
-    int m = live ? b.nl_cnt[g] : 0;
-    const uint32_t* row = b.nl + (size_t)g * b.cap;
+    int m = live ? b.nlf_cnt[g] : 0;
+    const uint32_t* row = b.nlf + (size_t)g * b.cap_f;
     double dh = 0, gb = 0, lj = 0, hc = 0, gi0 = 0, gi1 = 0, gi2 = 0, dai = 0;
     for (int k = sub; k < m; k += L) {
         uint32_t e = row[k];
@@ -435,11 +550,13 @@ __global__ void kb_pair(Batch b) {
         if (r2 > kPairCut2) continue;
         const float4 qj = prm[j];
         const double Rj = Rs[j], Ej = Es[j];
-        PairTerms pt = pair_terms(r2, qi, qj, Ri, Rj, Ei, Ej);
-        dh += pt.dh;
-        gb += pt.gb;
-        lj += pt.lj;
-        hc += pt.hc;
+        if (j > i) {
+            PairTerms pt = pair_terms(r2, qi, qj, Ri, Rj, Ei, Ej);
+            dh += pt.dh;
+            gb += pt.gb;
+            lj += pt.lj;
+            hc += pt.hc;
+        }
         // derivatives (FP32 DH / GB, FP64 LJ / hard core)
         float r2f = (float)r2, r = sqrt_f(r2f), ir = rcp_f(r);
         float qp = qi.x * qj.x;
@@ -462,15 +579,10 @@ __global__ void kb_pair(Batch b) {
             if (kHardScale * xc * xc < kHardCap) dhc = kHardScale * 2.0 * xc * (-12.0 * p12 / rr);
         }
         double dr = ((double)ddh + (double)dgb + dlj + dhc) / rr;
-        double u0 = (pi[0] - x[4 * j]) * dr, u1 = (pi[1] - x[4 * j + 1]) * dr, u2 = (pi[2] - x[4 * j + 2]) * dr;
-        gi0 += u0;
-        gi1 += u1;
-        gi2 += u2;
-        atomicAdd(&G[3 * j], -u0);
-        atomicAdd(&G[3 * j + 1], -u1);
-        atomicAdd(&G[3 * j + 2], -u2);
+        gi0 += (pi[0] - x[4 * j]) * dr;
+        gi1 += (pi[1] - x[4 * j + 1]) * dr;
+        gi2 += (pi[2] - x[4 * j + 2]) * dr;
         dai += (double)dD * (double)qj.w;
-        atomicAdd(&dEda[j], (double)dD * (double)qi.w);
     }
     dh = group_sum<L>(dh);
     gb = group_sum<L>(gb);
@@ -486,10 +598,11 @@ __global__ void kb_pair(Batch b) {
         part[1] = gb;
         part[2] = lj;
         part[3] = hc;
-        atomicAdd(&G[3 * i], gi0);
-        atomicAdd(&G[3 * i + 1], gi1);
-        atomicAdd(&G[3 * i + 2], gi2);
-        atomicAdd(&dEda[i], dai);
+        // first writer of this atom's gradient and dE/da in the step
+        b.Gn[3 * (size_t)g] = gi0;
+        b.Gn[3 * (size_t)g + 1] = gi1;
+        b.Gn[3 * (size_t)g + 2] = gi2;
+        b.dEda[g] = dai;
     }
 }
 
@@ -534,7 +647,9 @@ __device__ __forceinline__ float dhct(float r, float ri, float rj) {
     return -dL * iL * iL + iU * iU + dB * lg + B * (dL * iL - iU);
 }
 
-// Born chain rule: a = 1/(1/R* - S/2) while a < 1/2, da/dS = a^2/2
+// Born chain rule: a = 1/(1/R* - S/2) while a < 1/2, da/dS = a^2/2; over the
+// full rows each wavefront adds only its own atom's share (both HCT
+// directions of each of its pairs), so there are no atomics
 __global__ void __launch_bounds__(kBlock) kb_bornchain(Batch b) {
     int g = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
     if (g >= b.n_atoms) return;
@@ -545,13 +660,12 @@ __global__ void __launch_bounds__(kBlock) kb_bornchain(Batch b) {
     const float4* prm = b.prm + sy.aoff;
     const double* A = b.a + sy.aoff;
     const double* dEda = b.dEda + sy.aoff;
-    double* G = b.Gn + 3 * (size_t)sy.aoff;
     double ai = A[i];
     double wi = ai < 0.5 ? dEda[i] * 0.5 * ai * ai : 0.0;
     const double pi[3] = {x[4 * i], x[4 * i + 1], x[4 * i + 2]};
     float ri = prm[i].y;
-    int m = b.nl_cnt[g];
-    const uint32_t* row = b.nl + (size_t)g * b.cap;
+    int m = b.nlf_cnt[g];
+    const uint32_t* row = b.nlf + (size_t)g * b.cap_f;
     double g0 = 0, g1 = 0, g2 = 0;
     for (int k = lane; k < m; k += kWave) {
         int j = (int)(row[k] & kIdxMask);
@@ -561,21 +675,18 @@ __global__ void __launch_bounds__(kBlock) kb_bornchain(Batch b) {
         double r2 = dist2_exact(pi, x + 4 * j), r = sqrt(r2);
         float rf = (float)r, rj = prm[j].y;
         double d = (wi * (double)dhct(rf, ri, rj) + wj * (double)dhct(rf, rj, ri)) / r;
-        double u0 = (pi[0] - x[4 * j]) * d, u1 = (pi[1] - x[4 * j + 1]) * d, u2 = (pi[2] - x[4 * j + 2]) * d;
-        g0 += u0;
-        g1 += u1;
-        g2 += u2;
-        atomicAdd(&G[3 * j], -u0);
-        atomicAdd(&G[3 * j + 1], -u1);
-        atomicAdd(&G[3 * j + 2], -u2);
+        g0 += (pi[0] - x[4 * j]) * d;
+        g1 += (pi[1] - x[4 * j + 1]) * d;
+        g2 += (pi[2] - x[4 * j + 2]) * d;
     }
     g0 = wave_sum(g0);
     g1 = wave_sum(g1);
     g2 = wave_sum(g2);
     if (lane == 0) {
-        atomicAdd(&G[3 * i], g0);
-        atomicAdd(&G[3 * i + 1], g1);
-        atomicAdd(&G[3 * i + 2], g2);
+        double* G = b.Gn + 3 * (size_t)g;
+        atomicAdd(&G[0], g0);
+        atomicAdd(&G[1], g1);
+        atomicAdd(&G[2], g2);
     }
 }
 
@@ -795,6 +906,7 @@ hipError_t launch_batch_step(const Batch& b, int init, hipStream_t st) {
     kb_scan<<<b.n_sys, 1024, 0, st>>>(b);
     kb_place<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
     kb_rank<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
+    kb_stage<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);
     kb_nl<<<blocks_for(N, kWavesPerBlock), kBlock, 0, st>>>(b);
     kb_born<<<blocks_for(N, kWavesPerBlock), kBlock, 0, st>>>(b);
     kb_born_fin<<<blocks_for(N, kBlock), kBlock, 0, st>>>(b);

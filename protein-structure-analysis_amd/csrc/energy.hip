@@ -183,7 +183,9 @@ __global__ void __launch_bounds__(kBlock) k_nl_fill(const double* __restrict__ p
     }
     if (lane == 0) {
         cnt[i] = count;
-        atomicMax(&flags[1], count);
+        // same-address atomics serialise in L2: only a longer row issues one
+        if (count > __hip_atomic_load(&flags[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&flags[1], count);
         if (count > cap) atomicOr(&flags[0], 1);
     }
 }

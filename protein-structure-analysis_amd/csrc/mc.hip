@@ -754,14 +754,28 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     // counted as a rebuild) and after every accepted move that drifted an
     // atom past 1 A (needs_rebuild, :345-351)
     __shared__ int pending;
-    if (tid == 0) pending = cs->status == CS_REBUILD ? 1 : 0;
+    if (tid == 0) pending = cs->status == CS_REBUILD ? 1 : (cs->rb_pending ? 2 : 0);
     __syncthreads();
     for (;;) {
+        // the reference checks needs_rebuild at the start of a step: a build
+        // owed by the run's last move waits for the next call's first step
+        if (pending == 2 && cs->steps_done >= cs->steps_target) {
+            __syncthreads();
+            if (tid == 0) {
+                cs->rb_pending = 1;
+                pending = 0;
+            }
+            __syncthreads();
+        }
         if (pending) {
             if (tid == 0) t_prev = wall_clock64();
             chain_rebuild(b, s, c, pending - 1, dhalf, flag, stage_cp, stage_ci, stage_rb, lds_stage);
             tick(7);
-            if (tid == 0) pending = 0;
+            if (tid == 0) {
+                pending = 0;
+                cs->rb_pending = 0;
+            }
+            __syncthreads();
         }
         if (cs->status != CS_RUN) break;
         if (tid == 0) t_prev = wall_clock64();

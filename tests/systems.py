@@ -28,6 +28,23 @@ def chain_ss():
     return synth._add_disulfides(synth.make_chain(len(_SS_SEQ), 21, seq=list(_SS_SEQ)), 2)
 
 
+@functools.lru_cache(maxsize=None)
+def copies(n_copies, n_ss=0, step=52.0):
+    """n_copies of the 300-residue chain, copy k shifted by k * step A along
+    x so neighbouring copies share an interface (atoms: 4,965 per copy), with
+    n_ss SG-SG pairs moved to 2.04 A across the complex."""
+    base = chain(300, 2)
+    parts = []
+    for k in range(n_copies):
+        c = dict(base)
+        c["xyz"] = base["xyz"] + np.array([k * step, 0.0, 0.0])
+        parts.append(c)
+    sysd = synth.concat(parts, gap=2)
+    if n_ss:
+        sysd = synth._add_disulfides(sysd, n_ss)
+    return sysd
+
+
 def bridge():
     """tests/bridge_test.py:83-85 of the reference: three particles."""
     return {"xyz": np.array([[0.0, 0, 0], [5.0, 0, 0], [10.0, 0, 0]]),

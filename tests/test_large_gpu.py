@@ -1,0 +1,72 @@
+"""Systems above the LDS limits of the MC kernel (list builds staged in
+global memory above 5,632 atoms, per-step atom state in global memory above
+11,264 atoms) and a multi-chain complex with inter-chain disulfides, against
+the FP64 oracle: energy terms, forces and MC trajectories (same tolerances as
+the smaller systems)."""
+import numpy as np
+import pytest
+
+import systems as S
+from alma import _native as N
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KT = 0.0019872041 * 300.0
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+CASES = [("2x300+ss", lambda: S.copies(2, n_ss=2)), ("3x300", lambda: S.copies(3))]
+
+
+def _topo(s):
+    pairs = [tuple(int(v) for v in p) for p in s["ss_pairs"]]
+    return S.csr(S.topology(s, pairs)), pairs
+
+
+@pytest.mark.parametrize("name,make", CASES)
+def test_large_energy_and_forces(ctx, name, make):
+    s = make()
+    c, pairs = _topo(s)
+    assert len(c["ss_i"]) == len(pairs)
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    got = ctx.energy()
+    ref = O.energy_terms(s, c)
+    for k in O.TERMS:
+        if ref[k] == 0.0:
+            assert got[k] == 0.0, (name, k)
+        else:
+            assert abs(got[k] - ref[k]) <= 1e-5 * abs(ref[k]), (name, k, got[k], ref[k])
+    e, g, gt = ctx.forces()
+    E, G, GT = O.forces(s, c)
+    err = np.linalg.norm(g - G, axis=1)
+    mag = np.linalg.norm(G, axis=1)
+    assert np.all(err <= 1e-5 * (mag + np.sqrt(np.mean(mag ** 2)))), name
+
+
+@pytest.mark.parametrize("name,make", CASES)
+@pytest.mark.parametrize("mode,steps", [(N.MC_ENSEMBLE, 60), (N.MC_TORSION30, 40)])
+def test_large_trajectory_matches_oracle(ctx, name, make, mode, steps):
+    s = make()
+    c, _ = _topo(s)
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    gpu = ctx.mc_run(mode, steps, n_chains=2, temperature=KT, max_angle=0.12, seed=4321, trace=True)
+    orc = O.mc_run(s, c, steps, KT, 0.12, seed=4321, mode=mode, trace=True, l1=True)
+    gt, ot = gpu["trace"], orc["trace"]
+    assert np.array_equal(gt["kind"], ot["kind"]), name
+    assert np.array_equal(gt["accepted"], ot["accepted"]), name
+    tol = 1e-5 * ot["dE_l1"] + 1e-9
+    assert np.all(np.abs(gt["dE"] - ot["dE"]) <= tol), name
+    gs, os_ = gpu["stats"][0], orc["stats"]
+    assert gs["words"] == os_["words"] and gs["rebuilds"] == os_["rebuilds"], name
+    assert abs(gs["cur_e"] - os_["cur_e"]) <= 1e-6 * max(1.0, abs(os_["cur_e"])), name
+    assert np.max(np.abs(gpu["xyz"][0] - orc["xyz"])) < 1e-8, name
+    # the second chain draws from seed + 1
+    o2 = O.mc_run(s, c, steps, KT, 0.12, seed=4322, mode=mode)
+    assert np.max(np.abs(gpu["xyz"][1] - o2["xyz"])) < 1e-8, name
