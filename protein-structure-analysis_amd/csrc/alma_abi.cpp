@@ -335,6 +335,8 @@ void free_chains(ChainBatch& b) {
     dev_free(b.half);
     dev_free(b.dhalf);
     dev_free(b.born_sum);
+    dev_free(b.sa_cur);
+    dev_free(b.sa_new);
     dev_free(b.part);
     dev_free(b.rec);
     dev_free(b.flag);
@@ -391,6 +393,8 @@ void alloc_chains(ChainBatch& b, const DevSys& s, int nch, int trace_steps) {
     dev_alloc(&b.half, N);
     dev_alloc(&b.dhalf, N);
     dev_alloc(&b.born_sum, N);
+    dev_alloc(&b.sa_cur, N);
+    dev_alloc(&b.sa_new, N);
     dev_alloc(&b.part, 5 * N);
     dev_alloc(&b.flag, N);
     dev_alloc(&b.nlh_cnt, N);
@@ -589,16 +593,19 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     if (std::getenv("ALMA_MC_PROFILE")) {
         ChainState& x = st[0];
         long long tot = 0;
-        for (int k = 0; k < 7; ++k) tot += x.clk[k];
+        for (int k = 0; k < 11; ++k) tot += k == 7 ? 0 : x.clk[k];
         std::fprintf(stderr,
                      "[alma mc] chain0 steps=%d torsion=%lld crank=%lld conc=%lld  us/step: select %.2f move %.2f "
                      "segments %.2f passA %.2f radii %.2f passB %.2f commit %.2f total %.2f  rebuild %.2f us x %lld\n",
                      x.steps_done, x.steps_by_kind[0], x.steps_by_kind[1], x.steps_by_kind[2],
                      x.clk[0] / 100.0 / x.steps_done, x.clk[1] / 100.0 / x.steps_done,
                      x.clk[2] / 100.0 / x.steps_done, x.clk[3] / 100.0 / x.steps_done,
-                     x.clk[4] / 100.0 / x.steps_done, x.clk[5] / 100.0 / x.steps_done,
+                     x.clk[4] / 100.0 / x.steps_done, (x.clk[5] + x.clk[8] + x.clk[9] + x.clk[10]) / 100.0 / x.steps_done,
                      x.clk[6] / 100.0 / x.steps_done, tot / 100.0 / x.steps_done,
                      x.rebuilds ? x.clk[7] / 100.0 / x.rebuilds : 0.0, x.rebuilds);
+        std::fprintf(stderr, "[alma mc] pass B us/step: cross pairs %.2f SASA %.2f dihedral+SS %.2f accept %.2f\n",
+                     x.clk[8] / 100.0 / x.steps_done, x.clk[9] / 100.0 / x.steps_done,
+                     x.clk[10] / 100.0 / x.steps_done, x.clk[5] / 100.0 / x.steps_done);
         long long nb = x.rebuilds + 1;
         std::fprintf(stderr, "[alma mc] list build us: grid %.1f cells %.1f fill %.1f born %.1f sasa %.1f | per-wave sums: "
                      "atom setup %.1f candidate scan %.1f full-row conversion %.1f\n",

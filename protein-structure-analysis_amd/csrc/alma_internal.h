@@ -118,7 +118,8 @@ struct ChainState {
     int status;
     int acc_win, tot_win;
     int mode;
-    long long clk[8];    // per-phase clock ticks (select, move, segments, pass A, radii, pass B, commit, list rebuild)
+    long long clk[11];   // per-phase clock ticks (select, move, segments, pass A, radii, pass B accept, commit,
+                         // list rebuild, pass B cross pairs, pass B SASA, pass B dihedral + SS)
     long long rclk[8];   // list-build phases (clear+grid, cells, fill, full rows, Born, SASA)
     long long steps_by_kind[3];
     // algorithmic work census: cross pairs (HCT change), in-cutoff pair
@@ -143,6 +144,8 @@ struct ChainBatch {
     double* born_sum = nullptr;
     double* part = nullptr;   // nch * 5n
     double* rec = nullptr;    // nch * (n_dih + n_ss)
+    double* sa_cur = nullptr; // nch * n SASA fold of each atom at the current coordinates
+    double* sa_new = nullptr; // nch * n the fold at a step's trial coordinates (atoms the step touches)
     int* flag = nullptr;      // nch * n atoms whose SASA fold a move touches
     uint32_t* nlh = nullptr;  // nch * n * cap_h half rows, cell-scan order, excl bit
     uint32_t* nlf = nullptr;  // nch * n * cap_f full rows, ascending index, excl bit

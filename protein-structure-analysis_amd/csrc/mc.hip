@@ -100,6 +100,16 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// FP64 atomic add through an LDS-typed pointer (ds_add_f64) when the array
+// lives in LDS, a global atomic otherwise
+__device__ __forceinline__ void atomic_add_at(double* p, double v, bool lds) {
+    if (lds)
+        __hip_atomic_fetch_add((__attribute__((address_space(3))) double*)p, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+        atomicAdd(p, v);
+}
+
 // Exclusion test against atom i's symmetric exclusion row cached in LDS
 // (ex[0..ne), ascending; rows longer than kExRow fall back to a binary
 // search in global memory).
@@ -507,30 +517,40 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
     }
     wg_sync_global();
     rtick(2);
-    // HCT sums over every half-row entry, both directions
+    // HCT sums (born_radii :2024-2039: every half-row pair, both directions)
+    // into LDS when the stage lives there (it is idle now)
+    double* lsum = rbits_lds ? (double*)cp : sum;
+    if (rbits_lds) {
+        __syncthreads();
+        for (int i = tid; i < n; i += kMcBlock) lsum[i] = 0.0;
+        __syncthreads();
+    }
     for (int i = wave; i < n && grid_ok; i += kMcWaves) {
         const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
-        float ri = s.prm[i].y;
-        int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
+        const float ri = s.prm[i].y;
+        const int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
         const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
         double acc = 0.0;
         for (int k = lane; k < m; k += kWave) {
-            int j = (int)(row[k] & kIdxMask);
-            float r2 = (float)dist2_exact(pi, pos + 4 * j), r = sqrtf(r2);
-            float rj = s.prm[j].y;
+            const int j = (int)(row[k] & kIdxMask);
+            const float r2 = (float)dist2_exact(pi, pos + 4 * j), r = sqrtf(r2);
+            const float rj = s.prm[j].y;
             acc += (double)hct_f(r, r2, ri, rj);
-            atomicAdd(&sum[j], (double)hct_f(r, r2, rj, ri));
+            atomic_add_at(&lsum[j], (double)hct_f(r, r2, rj, ri), rbits_lds);
         }
         acc = wave_sum(acc);
         if (lane == 0) {
-            atomicAdd(&sum[i], acc);
+            atomic_add_at(&lsum[i], acc, rbits_lds);
             b.half[R1(b, c) + i] = acc;
         }
     }
-    wg_sync_global();
+    if (rbits_lds)
+        __syncthreads();
+    else
+        wg_sync_global();
     for (int i = tid; i < n; i += kMcBlock) {
         size_t o = R1(b, c) + i;
-        b.a[o] = 1.0 / fmax(1.0 / s.rstar[i] - 0.5 * sum[i], 2.0);
+        b.a[o] = 1.0 / fmax(1.0 / s.rstar[i] - 0.5 * lsum[i], 2.0);
         for (int d = 0; d < 4; ++d) ref[4 * i + d] = pos[4 * i + d];
     }
     __syncthreads();
@@ -543,11 +563,11 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
     // once, after the run, by the batched chain_energy kernels.
     for (int i = wave; i < n && grid_ok; i += kMcWaves) {
         const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
-        double ri = (double)s.prm[i].y + kProbe;
-        int ms = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
+        const double ri = (double)s.prm[i].y + kProbe;
+        const int ms = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
         int clamped;
-        double sa = sasa_fold_wave(pi, pos, s.prm, ri, b.nls + (R1(b, c) + i) * (size_t)b.cap_s, ms, &clamped);
-        if (lane == 0) b.part[5 * (R1(b, c) + i) + 4] = kGammaSA * fmax(0.0, sa);
+        const double sa = sasa_fold_wave(pi, pos, s.prm, ri, b.nls + (R1(b, c) + i) * (size_t)b.cap_s, ms, &clamped);
+        if (lane == 0) b.sa_cur[R1(b, c) + i] = sa;
     }
     // the stage aliases the step's LDS arrays: clear them for the next step
     __syncthreads();
@@ -1153,6 +1173,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 else if (in_n)
                     acc += pair_energy(r2n, ni, nj, Ri, Rj, ei, ej);
             });
+            tick(8);
             // SASA: compact the marked atoms, then one wavefront per atom
             // gathers 64 row entries at a time, ballots the contacts (old and
             // new geometry) and folds them in row order from shuffled caps
@@ -1166,11 +1187,12 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 for (int q = wave; q < na; q += kMcWaves) {
                     const int i = aff[q];
                     const bool gi = group(i) != 0;
-                    const double* pio = pos + 4 * i;
-                    const double* pin = gi ? posn + 4 * i : pio;
+                    const double* pin = gi ? posn + 4 * i : pos + 4 * i;
                     const double ri = (double)s.prm[i].y + kProbe;
                     const double k2 = M_PI * ri;
-                    double sa_o = 4.0 * M_PI * ri * ri, sa_n = sa_o;
+                    // the fold at the current coordinates is kept per atom
+                    // (set at the list build, replaced on every accept)
+                    double sa_n = 4.0 * M_PI * ri * ri;
                     int m;
                     const uint32_t* row;
                     if (far_move) {   // trial move beyond the sub-row's reach: whole half row
@@ -1180,46 +1202,36 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                         m = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
                         row = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
                     }
-                    cnt_sasa += lane == 0 ? 2 * m : 0;
+                    cnt_sasa += lane == 0 ? m : 0;
                     for (int base = 0; base < m; base += kWave) {
                         const int k = base + lane;
-                        double cap_o = 0.0, cap_n = 0.0;
-                        bool co = false, cn = false;
+                        double cap_n = 0.0;
+                        bool cn = false;
                         if (k < m) {
                             const int j = (int)(row[k] & kIdxMask);
                             const double dc = ri + (double)s.prm[j].y + kProbe;
-                            const double* pjo = pos + 4 * j;
-                            const double* pjn = group(j) ? posn + 4 * j : pjo;
-                            double r2 = dist2_exact(pio, pjo);
-                            if (r2 < dc * dc) {
-                                double r = (double)sqrtf((float)r2);
-                                co = r < dc;
-                                cap_o = k2 * (dc - r);
-                            }
-                            r2 = dist2_exact(pin, pjn);
+                            const double* pjn = group(j) ? posn + 4 * j : pos + 4 * j;
+                            const double r2 = dist2_exact(pin, pjn);
                             if (r2 < dc * dc) {
                                 double r = (double)sqrtf((float)r2);
                                 cn = r < dc;
                                 cap_n = k2 * (dc - r);
                             }
                         }
-                        uint64_t mo = __ballot(co), mn = __ballot(cn);
-                        while (mo) {
-                            int bit = __ffsll((unsigned long long)mo) - 1;
-                            double cp = __shfl(cap_o, bit, kWave);
-                            sa_o -= fmin(sa_o * 0.85, cp);
-                            mo &= mo - 1ull;
-                        }
+                        uint64_t mn = __ballot(cn);
                         while (mn) {
                             int bit = __ffsll((unsigned long long)mn) - 1;
-                            double cp = __shfl(cap_n, bit, kWave);
-                            sa_n -= fmin(sa_n * 0.85, cp);
+                            sa_n -= fmin(sa_n * 0.85, lane_value(cap_n, bit));
                             mn &= mn - 1ull;
                         }
                     }
-                    if (lane == 0) acc_sa += kGammaSA * (fmax(0.0, sa_n) - fmax(0.0, sa_o));
+                    if (lane == 0) {
+                        acc_sa += kGammaSA * (fmax(0.0, sa_n) - fmax(0.0, b.sa_cur[R1(b, c) + i]));
+                        b.sa_new[R1(b, c) + i] = sa_n;
+                    }
                 }
             }
+            tick(9);
             if (s.has_topo) {
                 for (int k = tid; k < T.n_dih; k += kMcBlock) {
                     int q4[4] = {T.dih_a[k], T.dih_b[k], T.dih_c[k], T.dih_d[k]};
@@ -1227,13 +1239,10 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     for (int m = 0; m < 4; ++m) (group(q4[m]) ? any_m : any_f) = true;
                     if (!any_m || !any_f) continue;
                     cnt_dih += 1;
-                    double po[16], pn[16];
-                    for (int m = 0; m < 4; ++m)
-                        for (int d = 0; d < 4; ++d) {
-                            po[4 * m + d] = pos[4 * q4[m] + d];
-                            pn[4 * m + d] = group(q4[m]) ? posn[4 * q4[m] + d] : pos[4 * q4[m] + d];
-                        }
-                    double fo = dih_angle_d(po, 0, 1, 2, 3), fn = dih_angle_d(pn, 0, 1, 2, 3);
+                    const double* pn[4];
+                    for (int m = 0; m < 4; ++m) pn[m] = (group(q4[m]) ? posn : pos) + 4 * q4[m];
+                    double fo = dih_angle_p(pos + 4 * q4[0], pos + 4 * q4[1], pos + 4 * q4[2], pos + 4 * q4[3]);
+                    double fn = dih_angle_p(pn[0], pn[1], pn[2], pn[3]);
                     for (int m = T.dih_toff[k]; m < T.dih_toff[k + 1]; ++m) {
                         double nn = (double)T.dih_n[m];
                         acc_dh += T.dih_v2[m] * (cos(nn * fn - T.dih_gamma[m]) - cos(nn * fo - T.dih_gamma[m]));
@@ -1249,6 +1258,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     acc_ss += kKss * (drn * drn - dro * dro);
                 }
             }
+            tick(10);
             acc = wave_sum(acc);
             acc_sa = wave_sum(acc_sa);
             acc_dh = wave_sum(acc_dh);
@@ -1294,6 +1304,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             for (int i = tid; i < n; i += kMcBlock) {
                 if (!flag[i]) continue;
                 if (mv.accept) half[i] += dhalf[i];
+                if (mv.accept && (flag[i] & 2)) b.sa_cur[R1(b, c) + i] = b.sa_new[R1(b, c) + i];
                 dhalf[i] = 0.0;
                 flag[i] = 0;
             }

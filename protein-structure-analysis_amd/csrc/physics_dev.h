@@ -179,6 +179,15 @@ __device__ __forceinline__ double pair_delta(double r2o, double r2n, float4 po_i
 // the FP64 d^2) removes min(0.85 sa, pi R_i (dc - r)).  Lanes test 64 entries
 // at once; the order-dependent fold then walks the contacts of the chunk in
 // order through wave shuffles, so the fold is bit-identical to a serial one.
+// Value of v in lane src (wave-uniform src): two v_readlane into scalar
+// registers, no LDS round trip as a shuffle takes.  The folds below are
+// serial dependency chains, so this latency is on their critical path.
+__device__ __forceinline__ double lane_value(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+
 struct SasaContact {
     bool hit;
     int j;
@@ -215,7 +224,7 @@ __device__ inline double sasa_fold_wave(const double* pi, const double* pos, con
         while (mask) {
             int b = __ffsll((unsigned long long)mask) - 1;
             mask &= mask - 1;
-            double cap = __shfl(c.cap, b, kWave);
+            double cap = lane_value(c.cap, b);
             if (sa * 0.85 < cap) {
                 sa -= sa * 0.85;
                 ++clamped;
@@ -226,6 +235,72 @@ __device__ inline double sasa_fold_wave(const double* pi, const double* pos, con
     }
     *n_clamped = clamped;
     return sa;
+}
+
+// Two atoms' folds by one wavefront: the row entries and partner gathers of
+// both are issued before either is used (indices clamped into the rows, so
+// every load is in bounds and the code is branch-free up to the folds), and
+// the two serial folds are stepped together.  Same arithmetic as
+// sasa_fold_wave for each atom.
+__device__ inline void sasa_fold_wave2(const double* pia, const double* pib, const double* pos, const float4* prm,
+                                       double ria, double rib, int ia, int ib, const uint32_t* rowa, int ma,
+                                       const uint32_t* rowb, int mb, double* sa_a, double* sa_b) {
+    const int lane = lane_id();
+    const double k2a = 2.0 * M_PI * ria * ria / (2.0 * ria), k2b = 2.0 * M_PI * rib * rib / (2.0 * rib);
+    double sa = 4.0 * M_PI * ria * ria, sb = 4.0 * M_PI * rib * rib;
+    const int mm = ma > mb ? ma : mb;
+    for (int base = 0; base < mm; base += kWave) {
+        const int k = base + lane;
+        const bool va = k < ma, vb = k < mb;
+        const int ja = ma > 0 ? (int)(rowa[va ? k : ma - 1] & kIdxMask) : ia;
+        const int jb = mb > 0 ? (int)(rowb[vb ? k : mb - 1] & kIdxMask) : ib;
+        const double* qa = pos + 4 * ja;
+        const double* qb = pos + 4 * jb;
+        const double xa = qa[0], ya = qa[1], za = qa[2], xb = qb[0], yb = qb[1], zb = qb[2];
+        const float rja = prm[ja].y, rjb = prm[jb].y;
+        double capa = 0.0, capb = 0.0;
+        bool ha = false, hb = false;
+        {
+#pragma clang fp contract(off)
+            const double dxa = pia[0] - xa, dya = pia[1] - ya, dza = pia[2] - za;
+            const double dxb = pib[0] - xb, dyb = pib[1] - yb, dzb = pib[2] - zb;
+            const double r2a = dxa * dxa + dya * dya + dza * dza, r2b = dxb * dxb + dyb * dyb + dzb * dzb;
+            const double dca = ria + (double)rja + kProbe, dcb = rib + (double)rjb + kProbe;
+            if (va && r2a < dca * dca) {
+                const double r = (double)sqrtf((float)r2a);
+                ha = r < dca;
+                capa = k2a * (dca - r);
+            }
+            if (vb && r2b < dcb * dcb) {
+                const double r = (double)sqrtf((float)r2b);
+                hb = r < dcb;
+                capb = k2b * (dcb - r);
+            }
+        }
+        uint64_t ma_ = __ballot(ha), mb_ = __ballot(hb);
+        while (ma_ | mb_) {
+            if (ma_) {
+                const int bit = __ffsll((unsigned long long)ma_) - 1;
+                const double cp = lane_value(capa, bit);
+                if (sa * 0.85 < cp)
+                    sa -= sa * 0.85;
+                else
+                    sa -= cp;
+                ma_ &= ma_ - 1ull;
+            }
+            if (mb_) {
+                const int bit = __ffsll((unsigned long long)mb_) - 1;
+                const double cp = lane_value(capb, bit);
+                if (sb * 0.85 < cp)
+                    sb -= sb * 0.85;
+                else
+                    sb -= cp;
+                mb_ &= mb_ - 1ull;
+            }
+        }
+    }
+    *sa_a = sa;
+    *sa_b = sb;
 }
 
 // Adjoint of the fold for a wavefront: each unclamped contact carries
@@ -245,7 +320,7 @@ __device__ inline void sasa_adjoint_wave(const double* pi, const double* pos, co
         while (walk) {
             int b = __ffsll((unsigned long long)walk) - 1;
             walk &= walk - 1;
-            double cap = __shfl(c.cap, b, kWave);
+            double cap = lane_value(c.cap, b);
             if (sa * 0.85 < cap) {
                 sa -= sa * 0.85;
                 cl |= 1ull << b;
@@ -273,12 +348,14 @@ __device__ inline void sasa_adjoint_wave(const double* pi, const double* pos, co
 // ── dihedral / disulfide (FP64) ─────────────────────────────────────────────
 // dihedral_angle :1871-1885 (atan2 of the reference's non-IUPAC pair),
 // dihedral_e :1912-1921, ss_e :1965-1974.
-__device__ inline double dih_angle_d(const double* pos, int a, int b, int c, int d) {
+// the angle from four coordinate pointers (each may point into a different
+// array: current or trial coordinates)
+__device__ inline double dih_angle_p(const double* pa, const double* pb, const double* pc, const double* pd) {
     double b1[3], b2[3], b3[3];
     for (int k = 0; k < 3; ++k) {
-        b1[k] = pos[4 * a + k] - pos[4 * b + k];
-        b2[k] = pos[4 * c + k] - pos[4 * b + k];
-        b3[k] = pos[4 * d + k] - pos[4 * c + k];
+        b1[k] = pa[k] - pb[k];
+        b2[k] = pc[k] - pb[k];
+        b3[k] = pd[k] - pc[k];
     }
     double n1x = b1[1] * b2[2] - b1[2] * b2[1], n1y = b1[2] * b2[0] - b1[0] * b2[2],
            n1z = b1[0] * b2[1] - b1[1] * b2[0];
@@ -287,6 +364,10 @@ __device__ inline double dih_angle_d(const double* pos, int a, int b, int c, int
     double m1x = n1y * b2[2] - n1z * b2[1], m1y = n1z * b2[0] - n1x * b2[2],
            m1z = n1x * b2[1] - n1y * b2[0];
     return atan2(m1x * n2x + m1y * n2y + m1z * n2z, n1x * n2x + n1y * n2y + n1z * n2z);
+}
+
+__device__ inline double dih_angle_d(const double* pos, int a, int b, int c, int d) {
+    return dih_angle_p(pos + 4 * a, pos + 4 * b, pos + 4 * c, pos + 4 * d);
 }
 
 
