@@ -387,37 +387,9 @@ __global__ void __launch_bounds__(kBlock) kb_nl(Batch b) {
 #pragma unroll
     for (int q = 0; q < 27; ++q) offs[q] = __builtin_amdgcn_readlane(off_c, q);
 
-    // exclusions of i: offsets -64..63 as bits of (wlo, whi), the rest flagged
+    // exclusions of i: offsets -64..63 as a register window, the rest searched
     const int eb = b.exs_off[pr.excl_base + i], ee = b.exs_off[pr.excl_base + i + 1];
-    uint32_t wm[4] = {0u, 0u, 0u, 0u};
-    bool far = false;
-    for (int e0 = eb; e0 < ee; e0 += kWave) {
-        uint32_t m0 = 0u, m1 = 0u, m2 = 0u, m3 = 0u;
-        bool f = false;
-        if (e0 + lane < ee) {
-            const int o = b.exs_idx[e0 + lane] - i + 64;
-            if (o >= 0 && o < 128) {
-                const uint32_t bit = 1u << (o & 31);
-                m0 = (o >> 5) == 0 ? bit : 0u;
-                m1 = (o >> 5) == 1 ? bit : 0u;
-                m2 = (o >> 5) == 2 ? bit : 0u;
-                m3 = (o >> 5) == 3 ? bit : 0u;
-            } else {
-                f = true;
-            }
-        }
-        for (int o = 1; o < kWave; o <<= 1) {
-            m0 |= (uint32_t)__shfl_xor((int)m0, o, kWave);
-            m1 |= (uint32_t)__shfl_xor((int)m1, o, kWave);
-            m2 |= (uint32_t)__shfl_xor((int)m2, o, kWave);
-            m3 |= (uint32_t)__shfl_xor((int)m3, o, kWave);
-        }
-        wm[0] |= m0;
-        wm[1] |= m1;
-        wm[2] |= m2;
-        wm[3] |= m3;
-        far = far || __ballot(f) != 0ull;
-    }
+    const ExclWindow xw = excl_window(b.exs_idx, eb, ee, i);
 
     uint32_t* row = b.nl + (size_t)g * b.cap;
     uint32_t* rowf = b.nlf + (size_t)g * b.cap_f;
@@ -456,14 +428,7 @@ __global__ void __launch_bounds__(kBlock) kb_nl(Batch b) {
         const uint64_t mf = __ballot(keep), m = __ballot(up);
         if (keep) {
             uint32_t v = (uint32_t)j;
-            const int o = j - i + 64;
-            bool ex;
-            if (o >= 0 && o < 128) {
-                const uint32_t w = o < 32 ? wm[0] : o < 64 ? wm[1] : o < 96 ? wm[2] : wm[3];
-                ex = (w >> (o & 31)) & 1u;
-            }
-            else ex = far && in_sorted(b.exs_idx, eb, ee, j);
-            if (ex) v |= kExclBit;
+            if (excl_test(xw, b.exs_idx, eb, ee, i, j)) v |= kExclBit;
             int slot = countf + __popcll(mf & below);
             if (slot < b.cap_f) rowf[slot] = v;
             if (up) {

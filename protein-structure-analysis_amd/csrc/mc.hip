@@ -110,27 +110,7 @@ __device__ __forceinline__ void atomic_add_at(double* p, double v, bool lds) {
         atomicAdd(p, v);
 }
 
-// Exclusion test against atom i's symmetric exclusion row cached in LDS
-// (ex[0..ne), ascending; rows longer than kExRow fall back to a binary
-// search in global memory).
-constexpr int kExRow = 64;
 constexpr int kLdsCells = 1024;   // cell-start tables up to this size are read from LDS
-__device__ __forceinline__ bool excluded_row(const int* ex, int ne, const DevTopo& T, int i, int j) {
-    if (ne > kExRow) return in_sorted(T.exs_idx, T.exs_off[i], T.exs_off[i + 1], j);
-    if (ne == 0 || j < ex[0] || j > ex[ne - 1]) return false;
-    for (int t = 0; t < ne; ++t)
-        if (ex[t] == j) return true;
-    return false;
-}
-
-__device__ __forceinline__ int load_ex_row(const DevTopo& T, bool has_topo, int i, int* ex) {
-    if (!has_topo) return 0;
-    const int e0 = T.exs_off[i], ne = T.exs_off[i + 1] - e0;
-    if (ne <= kExRow)
-        for (int t = lane_id(); t < ne; t += kWave) ex[t] = T.exs_idx[e0 + t];
-    wave_sync_lds();
-    return ne;
-}
 
 // One wavefront per atom i over the staged coordinates: every partner j != i
 // within the list radius sets bit j of the wavefront's row bitmap, partners
@@ -144,7 +124,7 @@ __device__ __forceinline__ int load_ex_row(const DevTopo& T, bool has_topo, int 
 template <bool kLds>
 __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, int c, const Grid& gr,
                                           const int* cell_of, const int* start, const float4* cp, const int* ci,
-                                          uint32_t* rb, int* pref, int* ex, int& max_h, int& max_s, int& max_f,
+                                          uint32_t* rb, int* pref, int& max_h, int& max_s, int& max_f,
                                           int& ovf, long long& sum_h, long long& sum_s) {
     const int n = b.n;
     const int lane = lane_id(), wave = threadIdx.x / kWave;
@@ -160,7 +140,8 @@ __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, 
         const float pix = (float)(pid[0] - gr.lo[0]), piy = (float)(pid[1] - gr.lo[1]),
                     piz = (float)(pid[2] - gr.lo[2]);
         const float ri = s.prm[i].y;
-        const int ne = load_ex_row(T, topo, i, ex);
+        const int eb = topo ? T.exs_off[i] : 0, ee = topo ? T.exs_off[i + 1] : 0;
+        const ExclWindow xw = excl_window(T.exs_idx, eb, ee, i);
         for (int w = lane; w < W; w += kWave) {
             if (kLds)
                 ((lds_u32*)rb)[w] = 0u;
@@ -245,7 +226,7 @@ __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, 
                             const uint64_t below = (1ull << lane) - 1ull;
                             if (up) {
                                 uint32_t v = (uint32_t)j;
-                                if (topo && excluded_row(ex, ne, T, i, j)) v |= kExclBit;
+                                if (excl_test(xw, T.exs_idx, eb, ee, i, j)) v |= kExclBit;
                                 int slot = nh + __popcll(m & below);
                                 if (slot < b.cap_h) rowh[slot] = v;
                                 if (near) {
@@ -323,7 +304,7 @@ __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, 
             }
             int j = (lo << 5) + bit;
             uint32_t v = (uint32_t)j;
-            if (topo && excluded_row(ex, ne, T, i, j)) v |= kExclBit;
+            if (excl_test(xw, T.exs_idx, eb, ee, i, j)) v |= kExclBit;
             rowf[t] = v;
         }
         t_conv += wall_clock64() - tc;
@@ -370,9 +351,7 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
     const DevTopo& T = s.topo;
     __shared__ double slo[3][kMcWaves], shi[3][kMcWaves];
     __shared__ int sbuf[kMcBlock];
-    __shared__ int sex[kMcWaves][kExRow];
     __shared__ int sstart[kLdsCells + 1];
-    int* ex = sex[wave];
     long long t0 = tid == 0 ? wall_clock64() : 0;
     auto rtick = [&](int ph) {
         if (tid == 0) {
@@ -501,9 +480,9 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
     long long sum_h = 0, sum_s = 0;
     if (grid_ok) {
         if (rbits_lds && lds_cells)
-            fill_rows<true>(b, s, c, gr, cell_of, sstart, cp, ci, rb, pref, ex, max_h, max_s, max_f, ovf, sum_h, sum_s);
+            fill_rows<true>(b, s, c, gr, cell_of, sstart, cp, ci, rb, pref, max_h, max_s, max_f, ovf, sum_h, sum_s);
         else
-            fill_rows<false>(b, s, c, gr, cell_of, start, cp, ci, rb, pref, ex, max_h, max_s, max_f, ovf, sum_h,
+            fill_rows<false>(b, s, c, gr, cell_of, start, cp, ci, rb, pref, max_h, max_s, max_f, ovf, sum_h,
                              sum_s);
     }
     (void)topo;

@@ -34,6 +34,60 @@ __device__ __forceinline__ float sqrt_f(float x) { return __builtin_amdgcn_sqrtf
 __device__ __forceinline__ float exp_f(float x) { return __expf(x); }
 __device__ __forceinline__ float log_f(float x) { return __logf(x); }
 
+// Exclusions of atom i as a wave-uniform window: partners at index offsets
+// -64..63 are bits of w[0..3] (scalar registers), the rest ("far", e.g. a
+// disulfide partner) are searched in the sorted exclusion row.  Built once
+// per atom by the whole wavefront.
+struct ExclWindow {
+    uint32_t w[4];
+    bool far;
+};
+
+__device__ __forceinline__ ExclWindow excl_window(const int* __restrict__ idx, int eb, int ee, int i) {
+    const int lane = lane_id();
+    uint32_t m0 = 0u, m1 = 0u, m2 = 0u, m3 = 0u;
+    bool f = false;
+    for (int e0 = eb; e0 < ee; e0 += kWave) {
+        if (e0 + lane < ee) {
+            const int o = idx[e0 + lane] - i + 64;
+            if (o >= 0 && o < 128) {
+                const uint32_t bit = 1u << (o & 31);
+                m0 |= (o >> 5) == 0 ? bit : 0u;
+                m1 |= (o >> 5) == 1 ? bit : 0u;
+                m2 |= (o >> 5) == 2 ? bit : 0u;
+                m3 |= (o >> 5) == 3 ? bit : 0u;
+            } else {
+                f = true;
+            }
+        }
+    }
+    for (int o = 1; o < kWave; o <<= 1) {
+        m0 |= (uint32_t)__shfl_xor((int)m0, o, kWave);
+        m1 |= (uint32_t)__shfl_xor((int)m1, o, kWave);
+        m2 |= (uint32_t)__shfl_xor((int)m2, o, kWave);
+        m3 |= (uint32_t)__shfl_xor((int)m3, o, kWave);
+    }
+    ExclWindow x;
+    x.w[0] = __builtin_amdgcn_readfirstlane(m0);
+    x.w[1] = __builtin_amdgcn_readfirstlane(m1);
+    x.w[2] = __builtin_amdgcn_readfirstlane(m2);
+    x.w[3] = __builtin_amdgcn_readfirstlane(m3);
+    x.far = __ballot(f) != 0ull;
+    return x;
+}
+
+__device__ __forceinline__ bool in_sorted(const int* __restrict__ v, int b, int e, int key);
+
+__device__ __forceinline__ bool excl_test(const ExclWindow& x, const int* __restrict__ idx, int eb, int ee, int i,
+                                          int j) {
+    const int o = j - i + 64;
+    if (o >= 0 && o < 128) {
+        const uint32_t w = o < 32 ? x.w[0] : o < 64 ? x.w[1] : o < 96 ? x.w[2] : x.w[3];
+        return (w >> (o & 31)) & 1u;
+    }
+    return x.far && in_sorted(idx, eb, ee, j);
+}
+
 // Reference FP64 squared distance, no contraction into FMA (the reference's
 // d^2 < 196 and d^2 > 144 decisions must be reproduced exactly).
 __device__ __forceinline__ double dist2_exact(const double* pi, const double* pj) {
