@@ -22,7 +22,14 @@ def ctx():
     c.close()
 
 
-CASES = [("2x300+ss", lambda: S.copies(2, n_ss=2)), ("3x300", lambda: S.copies(3))]
+# the last case has the shape of the 4 x 500-residue complex (~31,000 atoms,
+# 12 disulfides): six 300-residue chains on a 3 x 2 grid with ~4-5 A
+# interfaces, 29,766 atoms after the disulfide thiol H are removed
+def _complex():
+    return S.copies(6, n_ss=12, step=58.0, row=3, step_y=66.0)
+
+
+CASES = [("2x300+ss", lambda: S.copies(2, n_ss=2)), ("3x300", lambda: S.copies(3)), ("6x300+12ss", _complex)]
 
 
 def _topo(s):
@@ -70,3 +77,28 @@ def test_large_trajectory_matches_oracle(ctx, name, make, mode, steps):
     # the second chain draws from seed + 1
     o2 = O.mc_run(s, c, steps, KT, 0.12, seed=4322, mode=mode)
     assert np.max(np.abs(gpu["xyz"][1] - o2["xyz"])) < 1e-8, name
+
+
+def test_complex_mc_segments_with_forces(ctx):
+    """The large-complex workload: MC segments of 10 steps with a force
+    evaluation at each segment's end state, both against the oracle."""
+    s = _complex()
+    c, _ = _topo(s)
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    xyz = s["xyz"]
+    for seg in range(2):
+        ctx.set_positions(xyz)
+        gpu = ctx.mc_run(N.MC_ENSEMBLE, 10, n_chains=1, temperature=KT, max_angle=0.12, seed=900 + seg, trace=True)
+        cur = dict(s, xyz=xyz)
+        orc = O.mc_run(cur, c, 10, KT, 0.12, seed=900 + seg, mode=N.MC_ENSEMBLE, trace=True, l1=True)
+        assert np.array_equal(gpu["trace"]["accepted"], orc["trace"]["accepted"]), seg
+        assert np.max(np.abs(gpu["xyz"][0] - orc["xyz"])) < 1e-8, seg
+        xyz = gpu["xyz"][0]
+        ctx.set_positions(xyz)
+        e, g, gt = ctx.forces()
+        E, G, GT = O.forces(dict(s, xyz=xyz), c)
+        assert abs(e - E) <= 1e-5 * abs(E), seg
+        err = np.linalg.norm(g - G, axis=1)
+        mag = np.linalg.norm(G, axis=1)
+        assert np.all(err <= 1e-5 * (mag + np.sqrt(np.mean(mag ** 2)))), seg
+        assert np.all(np.abs(gt - GT) <= 1e-5 * (np.abs(GT) + np.sqrt(np.mean(GT ** 2)))), seg
