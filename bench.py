@@ -58,9 +58,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--force-reps", type=int, default=200)
     ap.add_argument("--mala-steps", type=int, default=2000)
-    ap.add_argument("--pt-proteins", type=int, default=16)
-    ap.add_argument("--pt-steps", type=int, default=100)
+    ap.add_argument("--pt-proteins", type=int, default=64)
+    ap.add_argument("--pt-max-res", type=int, default=800)
+    ap.add_argument("--pt-steps", type=int, default=200)
+    ap.add_argument("--pt-exchange", type=int, default=100)
+    ap.add_argument("--pt-plain-steps", type=int, default=50)
     ap.add_argument("--no-pt", action="store_true")
+    ap.add_argument("--complex-cycles", type=int, default=20,
+                    help="cycles of (10 MC steps + 1 force evaluation) on the 29,766-atom complex; 0 skips it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the runtime's copy kernel), "
                          "the reference for the FETCH_SIZE / WRITE_SIZE counter scale")
@@ -159,6 +164,11 @@ def main():
         pt = _pt_compare(s, args, PP)
         log("PT: batched MALA %.0f accepted/s, plain per-replica MC %.0f accepted/s"
             % (pt["batched_mala_pt"]["accepted_per_s"], pt["plain_per_replica_mc"]["accepted_per_s"]))
+    cx = None
+    if args.complex_cycles > 0:
+        cx = _complex_rate(s, args, PP, kt)
+        log("complex: %.1f MC steps/s with a force evaluation every 10 steps; %.1f dE evals/s over 64 chains"
+            % (cx["segment_mc_steps_per_s"], cx["chains64_evals_per_s"]))
     # the energy context is reused by the later calls: reload the bench system
     ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], csr)
 
@@ -188,6 +198,7 @@ def main():
         "forces": forces,
         "mala": mala,
         "parallel_tempering": pt,
+        "large_complex": cx,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -259,33 +270,54 @@ def _mala_rate(ctx, args, kt):
 
 
 def _segment(s, n_res):
-    """The first n_res residues of a chain, as its own protein."""
+    """The first n_res residues of a (possibly multi-chain) system, as its own
+    protein."""
     import numpy as np
-    first = int(np.min(s["res_idx"]))
-    keep = np.flatnonzero(np.asarray(s["res_idx"]) < first + n_res)
+    ridx = np.asarray(s["res_idx"])
+    first = np.unique(ridx)[:n_res]
+    keep = np.flatnonzero(np.isin(ridx, first))
     return {"xyz": s["xyz"][keep], "charge": s["charge"][keep], "rstar": s["rstar"][keep],
             "eps": s["eps"][keep], "resnames": [s["resnames"][k] for k in keep],
-            "atomnames": [s["atomnames"][k] for k in keep], "res_idx": np.asarray(s["res_idx"])[keep]}
+            "atomnames": [s["atomnames"][k] for k in keep], "res_idx": ridx[keep]}
+
+
+def _copies(s, n, step=58.0, row=0, step_y=66.0):
+    """n copies of a chain on a grid (x step, y step_y; `row` copies per row,
+    0 = one row), neighbouring copies 4-5 A apart at the defaults for the
+    300-residue bench chain."""
+    import numpy as np
+    from alma import synth
+    parts = []
+    for k in range(n):
+        gx, gy = (k % row, k // row) if row else (k, 0)
+        c = dict(s)
+        c["xyz"] = s["xyz"] + np.array([gx * step, gy * step_y, 0.0])
+        parts.append(c)
+    return synth.concat(parts, gap=2)
 
 
 def _pt_compare(s, args, PP):
     """Batched parallel tempering with MALA moves (all proteins and slots in
     one sequence of launches) against plain per-replica device MC (one launch
     per protein, its replicas as chains at their slot temperatures), in
-    accepted moves per second.  Proteins: the first L residues of the bench
-    chain, L ~ LogNormal(ln 180, 0.6) truncated to [40, 300]; 8 slots,
-    T_k = 280 (450/280)^(k/7) K."""
+    accepted moves per second.  Proteins: the first L residues of three
+    copies of the bench chain laid end to end (900 residues, chain breaks
+    every 300), L ~ LogNormal(ln 180, 0.6) truncated to [40, 800]; 8 slots,
+    T_k = 280 (450/280)^(k/7) K; exchanges every 100 steps."""
     import numpy as np
     from alma import _native as N
     rng = np.random.default_rng(7)
+    src = _copies(s, 3)
+    n_res_src = len(np.unique(src["res_idx"]))
+    hi = min(args.pt_max_res, n_res_src)
     sizes = []
     while len(sizes) < args.pt_proteins:
         v = int(round(rng.lognormal(np.log(180.0), 0.6)))
-        if 40 <= v <= args.residues:
+        if 40 <= v <= hi:
             sizes.append(v)
     prots, csrs = [], []
     for L in sizes:
-        p = _segment(s, L)
+        p = _segment(src, L)
         t = PP.BondTopology()
         t.build(p["resnames"], p["atomnames"], [int(v) for v in p["res_idx"]])
         prots.append(p)
@@ -294,16 +326,16 @@ def _pt_compare(s, args, PP):
     R = len(temps)
     ctx = N.Context(0)
     ctx.pt_set(prots, csrs, R, temps, [p["xyz"] for p in prots for _ in range(R)])
-    ctx.pt_run(5, exchange_every=10, mala_step=0.02, seed=3)
+    ctx.pt_run(5, exchange_every=args.pt_exchange, mala_step=0.02, seed=3)
     t0 = time.perf_counter()
-    r = ctx.pt_run(args.pt_steps, exchange_every=10, mala_step=0.02, seed=3)
+    r = ctx.pt_run(args.pt_steps, exchange_every=args.pt_exchange, mala_step=0.02, seed=3)
     dt = time.perf_counter() - t0
     tm = ctx.pt_timing()
     acc_pt = sum(x["accepted"] for x in r["stats"])
     moves_pt = sum(x["moves"] for x in r["stats"])
     swaps = r["swaps"].sum(axis=0)
     # plain device MC: per protein one launch, R chains at the slot temperatures
-    plain_steps = max(10, args.pt_steps)
+    plain_steps = args.pt_plain_steps
     acc_pl = 0
     moves_pl = 0
     t_pl = 0.0
@@ -316,15 +348,63 @@ def _pt_compare(s, args, PP):
         moves_pl += sum(x["moves"] for x in rr["stats"])
     ctx.close()
     return {
-        "proteins": len(prots), "slots": R, "residues": sizes,
+        "proteins": len(prots), "slots": R, "systems": len(prots) * R, "residues": sizes,
         "atoms_total": int(tm["atoms"]),
         "batched_mala_pt": {"steps": args.pt_steps, "accepted_per_s": acc_pt / dt, "accept_rate": acc_pt / max(moves_pt, 1),
-                            "exchange_every": 10, "swaps_attempted": int(swaps[0]), "swaps_accepted": int(swaps[1]),
+                            "exchange_every": args.pt_exchange, "swaps_attempted": int(swaps[0]),
+                            "swaps_accepted": int(swaps[1]), "seconds": dt,
                             "device_ms": tm["device_ms"], "pair_kernel_lanes": int(tm["pair_lanes"])},
         "plain_per_replica_mc": {"steps": plain_steps, "accepted_per_s": acc_pl / t_pl,
-                                 "accept_rate": acc_pl / max(moves_pl, 1), "mode": "generate_ensemble moves"},
-        "note": "residue counts capped at the bench chain's 300 (the protein set is cut from it)",
+                                 "accept_rate": acc_pl / max(moves_pl, 1), "seconds": t_pl,
+                                 "mode": "generate_ensemble moves"},
+        "note": "proteins are the first L residues of three copies of the bench chain laid end to end",
     }
+
+
+def _complex_rate(s, args, PP, kt):
+    """The large-complex workload: six copies of the bench chain on a 3 x 2
+    grid (4-5 A interfaces) with 12 disulfides (SG-SG at 2.04 A), 29,766
+    atoms.  One trajectory in cycles of 10 MC steps (generate_ensemble moves,
+    a fresh list build at each cycle's start as in the reference's call) plus
+    one force evaluation at the cycle's end state; and the throughput form,
+    64 chains x 100 steps in one launch."""
+    import numpy as np
+    from alma import _native as N
+    from alma import synth
+    cs = synth._add_disulfides(_copies(s, 6, row=3), 12)
+    t = PP.BondTopology()
+    t.build(cs["resnames"], cs["atomnames"], [int(v) for v in cs["res_idx"]])
+    for i, j in cs["ss_pairs"]:
+        t.add_disulfide(int(i), int(j))
+    c = _csr(t)
+    ctx = N.Context(0)
+    ctx.set_system(cs["xyz"], cs["charge"], cs["rstar"], cs["eps"], c)
+    ctx.mc_run(N.MC_ENSEMBLE, 10, n_chains=1, temperature=kt, seed=1)
+    ctx.forces()
+    xyz = cs["xyz"]
+    t0 = time.perf_counter()
+    t_mc = 0.0
+    for k in range(args.complex_cycles):
+        ctx.set_positions(xyz)
+        t1 = time.perf_counter()
+        r = ctx.mc_run(N.MC_ENSEMBLE, 10, n_chains=1, temperature=kt, seed=100 + k)
+        t_mc += time.perf_counter() - t1
+        xyz = r["xyz"][0]
+        ctx.set_positions(xyz)
+        ctx.forces()
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    r = ctx.mc_run(N.MC_ENSEMBLE, 100, n_chains=64, temperature=kt, seed=7)
+    d64 = time.perf_counter() - t1
+    ctx.close()
+    n = len(cs["xyz"])
+    return {"atoms": n, "disulfides": len(cs["ss_pairs"]), "cycles": args.complex_cycles,
+            "segment_mc_steps_per_s": 10 * args.complex_cycles / dt,
+            "segment_force_evals_per_s": args.complex_cycles / dt,
+            "mc_share_of_cycle": t_mc / dt,
+            "chains64_evals_per_s": 64 * 100 / d64,
+            "note": "cycle = set coordinates, 10 MC steps from a fresh list build, copy back, "
+                    "forces (energy + dE/dx + dE/dtheta) at the end state; host round trips included"}
 
 
 def _csr(t):

@@ -125,13 +125,16 @@ alma_status alma_mc_run(alma_ctx* ctx, const alma_mc_params* p, double* xyz_out,
 alma_status alma_mc_trace(alma_ctx* ctx, int32_t* kind, int32_t* bond, double* delta,
                           double* dE, int32_t* accepted, double* dE_parts);
 
-/* Measurement of the last alma_mc_run (no reference counterpart): out[11] =
+/* Measurement of the last alma_mc_run (no reference counterpart): out[12] =
  * {device MC kernel ms (HIP events on the context stream), MC launches,
  *  chain-time spent in list builds (ms, summed over chains; builds run
  *  inside the MC kernel), list rebuilds, cross pairs, in-cutoff pair
  *  evaluations, SASA row entries folded, moved atoms, boundary dihedral
  *  records, list-build half-row pairs, list-build SASA-row entries},
- *  summed over chains. */
+ *  summed over chains, then the number of times neighbour-row capacities or
+ *  the cell grid were grown during the run (a build that overflows stops its
+ *  chain, the capacities grow and the same build runs again, so results do
+ *  not depend on them). */
 alma_status alma_mc_timing(alma_ctx* ctx, double* out);
 
 /* ---- forces (no reference counterpart) ---- */
@@ -189,8 +192,10 @@ alma_status alma_pt_run(alma_ctx* ctx, const alma_pt_params* p, double* xyz_out,
 alma_status alma_pt_trace(alma_ctx* ctx, int32_t* bond, double* delta, double* dE, int32_t* accepted);
 /* Per-term energies (8 per system) of the current states, fresh evaluation. */
 alma_status alma_pt_energy(alma_ctx* ctx, double* terms);
-/* out[4] = {device ms of the last alma_pt_run, steps, atoms, lanes per row
- * of the pair kernel chosen by the autotuner}. */
+/* out[5] = {device ms of the last alma_pt_run, steps, atoms, lanes per row
+ * of the pair kernel chosen by the autotuner, capacity / cell-grid growths
+ * during the run (a block of steps in which a system outgrew its rows or
+ * cell table is rolled back and run again with larger ones)}. */
 alma_status alma_pt_timing(alma_ctx* ctx, double* out);
 
 #ifdef __cplusplus

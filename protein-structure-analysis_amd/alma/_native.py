@@ -250,10 +250,11 @@ class Context:
         return out
 
     def mc_timing(self) -> dict:
-        out = np.zeros(11)
+        out = np.zeros(12)
         self._check(lib().alma_mc_timing(self._h, _p(out, C.c_double)))
         keys = ("mc_kernel_ms", "mc_launches", "rebuild_chain_ms", "rebuilds", "cross_pairs",
-                "pair_evals", "sasa_entries", "moved_atoms", "dih_records", "build_pairs", "build_sasa")
+                "pair_evals", "sasa_entries", "moved_atoms", "dih_records", "build_pairs", "build_sasa",
+                "regrows")
         return dict(zip(keys, out.tolist()))
 
     def forces(self):
@@ -316,6 +317,6 @@ class Context:
         return out.reshape(S, 8)
 
     def pt_timing(self) -> dict:
-        out = np.zeros(4)
+        out = np.zeros(5)
         self._check(lib().alma_pt_timing(self._h, _p(out, C.c_double)))
-        return dict(zip(("device_ms", "steps", "atoms", "pair_lanes"), out.tolist()))
+        return dict(zip(("device_ms", "steps", "atoms", "pair_lanes", "regrows"), out.tolist()))

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -433,6 +434,72 @@ int pow2_at_least(int v) {
     return p;
 }
 
+int round64(long long v) { return (int)((v + 63) / 64 * 64); }
+
+// Widens a row array to a larger per-row capacity, keeping every row's
+// entries (the lists of chains that did not overflow stay valid).
+template <typename T>
+void widen_rows(T** p, size_t rows, int old_cap, int new_cap) {
+    T* q = nullptr;
+    HIP_TRY(hipMalloc((void**)&q, sizeof(T) * rows * (size_t)new_cap));
+    hipError_t e = hipMemcpy2D(q, sizeof(T) * new_cap, *p, sizeof(T) * old_cap, sizeof(T) * old_cap, rows,
+                               hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) {
+        hipFree(q);
+        HIP_TRY(e);
+    }
+    hipFree(*p);
+    *p = q;
+}
+
+// Chains whose list build overflowed a row capacity (CS_CAPACITY) or the
+// cell grid (CS_GRID) stopped at that build with their coordinates intact.
+// Grows the capacities to the counts the build measured (with margin) and
+// sets them to build again at the next launch; the build is then the same
+// one, so the trajectory does not depend on the capacities.  The repeated
+// build is not counted twice (CS_REBUILD builds are uncounted).
+void grow_for_overflow(ChainBatch& b, std::vector<ChainState>& st) {
+    const int nch = b.nch, n = b.n;
+    std::vector<int> fl(4 * (size_t)nch);
+    HIP_TRY(hipMemcpy(fl.data(), b.flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost));
+    std::vector<Grid> gr(nch);
+    HIP_TRY(hipMemcpy(gr.data(), b.grid, sizeof(Grid) * nch, hipMemcpyDeviceToHost));
+    int need_h = 0, need_f = 0, need_s = 0, need_cells = 0;
+    for (int c = 0; c < nch; ++c) {
+        if (st[c].status == CS_CAPACITY) {
+            need_h = std::max(need_h, fl[4 * c + 1]);
+            need_f = std::max(need_f, fl[4 * c + 2]);
+            need_s = std::max(need_s, fl[4 * c + 3]);
+        } else if (st[c].status == CS_GRID) {
+            need_cells = std::max(need_cells, gr[c].ncell);
+        }
+    }
+    const size_t rows = (size_t)nch * n;
+    if (need_h > b.cap_h) {
+        int cap = round64((long long)need_h * 5 / 4 + 64);
+        widen_rows(&b.nlh, rows, b.cap_h, cap);
+        b.cap_h = cap;
+    }
+    if (need_f > b.cap_f) {
+        int cap = round64((long long)need_f * 5 / 4 + 64);
+        widen_rows(&b.nlf, rows, b.cap_f, cap);
+        b.cap_f = cap;
+    }
+    if (need_s > b.cap_s) {
+        int cap = round64((long long)need_s * 5 / 4 + 64);
+        widen_rows(&b.nls, rows, b.cap_s, cap);
+        b.cap_s = cap;
+    }
+    if (need_cells > b.max_cells) {
+        // cell tables are rebuilt from scratch by every build: no copy
+        b.max_cells = need_cells + need_cells / 4 + 8;
+        dev_alloc(&b.cell_cnt, (size_t)nch * (b.max_cells + 1));
+        dev_alloc(&b.cell_start, (size_t)nch * (b.max_cells + 1));
+    }
+    for (int c = 0; c < nch; ++c)
+        if (st[c].status == CS_CAPACITY || st[c].status == CS_GRID) st[c].status = CS_REBUILD;
+}
+
 
 
 void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_stats* stats, double* snap_xyz,
@@ -457,6 +524,17 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     b.cap_h = (maxh * 3 / 2 + 63) / 64 * 64;
     b.cap_f = pow2_at_least(3 * maxh);
     b.cap_s = (std::max(64, maxh) + 63) / 64 * 64;
+    if (const char* tc = std::getenv("ALMA_MC_TEST_CAPS")) {
+        // testing: start from the given row capacity and cell-grid size, so
+        // that the growth path runs ("rows,cells")
+        int rows = 0, cells = 0;
+        if (std::sscanf(tc, "%d,%d", &rows, &cells) >= 1 && rows > 0) b.cap_h = b.cap_f = b.cap_s = round64(rows);
+        if (cells > 0) {
+            b.max_cells = cells;
+            dev_alloc(&b.cell_cnt, (size_t)nch * (b.max_cells + 1));
+            dev_alloc(&b.cell_start, (size_t)nch * (b.max_cells + 1));
+        }
+    }
     alloc_rows(b);
     // initial state of every chain
     {
@@ -471,6 +549,7 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     std::vector<HostMT> mt;
     ctx->t_mc_ms = ctx->t_rebuild_ms = 0.0;
     ctx->n_mc_launch = ctx->n_rebuild_launch = 0;
+    ctx->n_regrow = 0;
     if (!ctx->ev0) {
         HIP_TRY(hipEventCreate(&ctx->ev0));
         HIP_TRY(hipEventCreate(&ctx->ev1));
@@ -510,11 +589,16 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
         HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
         std::vector<int> run;
         bool dirty = false;
+        bool grow = false;
+        for (int c = 0; c < nch; ++c) grow |= st[c].status == CS_CAPACITY || st[c].status == CS_GRID;
+        if (grow) {
+            grow_for_overflow(b, st);
+            ctx->n_regrow += 1;
+            dirty = true;
+        }
         for (int c = 0; c < nch; ++c) {
             ChainState& x = st[c];
             if (x.status == CS_ERROR) throw Fail{ALMA_ERUNTIME, "MC chain ran out of random words"};
-            if (x.status == CS_GRID) throw Fail{ALMA_ERUNTIME, "chain expanded beyond its cell grid"};
-            if (x.status == CS_CAPACITY) throw Fail{ALMA_ERUNTIME, "neighbour rows exceeded their capacity"};
             if (x.status == CS_WORDS) {
                 // keep the unconsumed tail, append fresh words
                 int tail = x.wavail - x.cursor;
@@ -975,8 +1059,117 @@ void pt_set(alma_ctx* ctx, const alma_pt_systems* ps, const double* xyz) {
     batch_autotune(ctx);
     batch_init_eval(ctx);
     batch_check(ctx);
+    if (const char* tc = std::getenv("ALMA_PT_TEST_CAPS")) {
+        // testing: continue from the given row capacity so that the growth
+        // path of alma_pt_run runs
+        int rows = 0;
+        if (std::sscanf(tc, "%d", &rows) == 1 && rows > 0) {
+            b.cap = b.cap_f = round64(rows);
+            dev_alloc(&b.nl, (size_t)total * b.cap);
+            dev_alloc(&b.nlf, (size_t)total * b.cap_f);
+        }
+    }
     ctx->pt_step = 0;
     ctx->pt_ready = true;
+}
+
+// The state one batched step mutates, copied on the device.
+struct PtSnapshot {
+    double* pos = nullptr;
+    double* G = nullptr;
+    BState* st = nullptr;
+    int* slot_sys = nullptr;
+    long long* swaps = nullptr;
+    explicit PtSnapshot(const Batch& b) {
+        dev_alloc(&pos, 4 * (size_t)b.n_atoms);
+        dev_alloc(&G, 3 * (size_t)b.n_atoms);
+        dev_alloc(&st, b.n_sys);
+        dev_alloc(&slot_sys, b.n_sys);
+        dev_alloc(&swaps, 2 * (size_t)b.n_prot);
+    }
+    ~PtSnapshot() {
+        dev_free(pos);
+        dev_free(G);
+        dev_free(st);
+        dev_free(slot_sys);
+        dev_free(swaps);
+    }
+    static void copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+        HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+    }
+    void save(const Batch& b, hipStream_t s) {
+        copy(pos, b.pos, sizeof(double) * 4 * (size_t)b.n_atoms, s);
+        copy(G, b.G, sizeof(double) * 3 * (size_t)b.n_atoms, s);
+        copy(st, b.st, sizeof(BState) * b.n_sys, s);
+        copy(slot_sys, b.slot_sys, sizeof(int) * b.n_sys, s);
+        copy(swaps, b.swaps, sizeof(long long) * 2 * (size_t)b.n_prot, s);
+    }
+    void restore(Batch& b, hipStream_t s) {
+        copy(b.pos, pos, sizeof(double) * 4 * (size_t)b.n_atoms, s);
+        copy(b.G, G, sizeof(double) * 3 * (size_t)b.n_atoms, s);
+        copy(b.st, st, sizeof(BState) * b.n_sys, s);
+        copy(b.slot_sys, slot_sys, sizeof(int) * b.n_sys, s);
+        copy(b.swaps, swaps, sizeof(long long) * 2 * (size_t)b.n_prot, s);
+    }
+};
+
+// One batched step captured as a graph (fixed kernel sequence, the batch
+// descriptor by value: captured again whenever the descriptor changes).
+struct PtGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    PtGraph(alma_ctx* ctx, const Batch& b) {
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = launch_batch_step(b, 0, ctx->stream);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        HIP_TRY(le);
+        HIP_TRY(ce);
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            hipGraphDestroy(graph);
+            graph = nullptr;
+            HIP_TRY(ie);
+        }
+    }
+    ~PtGraph() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+    }
+};
+
+// After a rolled-back block: rows grow to the longest counted (with margin),
+// systems whose grid outgrew its cell table get a larger one (the tables are
+// rebuilt every step, so they are laid out afresh).
+void batch_grow(alma_ctx* ctx, int ovf) {
+    Batch& b = ctx->batch;
+    int fl[2];
+    HIP_TRY(hipMemcpy(fl, b.flags, sizeof(fl), hipMemcpyDeviceToHost));
+    if (ovf & 1) {
+        if (fl[0] > b.cap) {
+            b.cap = round64((long long)fl[0] * 5 / 4 + 64);
+            dev_alloc(&b.nl, (size_t)b.n_atoms * b.cap);
+        }
+        if (fl[1] > b.cap_f) {
+            b.cap_f = round64((long long)fl[1] * 5 / 4 + 64);
+            dev_alloc(&b.nlf, (size_t)b.n_atoms * b.cap_f);
+        }
+    }
+    if (ovf & 2) {
+        std::vector<Grid> gr(b.n_sys);
+        HIP_TRY(hipMemcpy(gr.data(), b.grid, sizeof(Grid) * b.n_sys, hipMemcpyDeviceToHost));
+        std::vector<BSys>& sys = ctx->h_bsys;
+        long long coff = 0;
+        for (int s = 0; s < b.n_sys; ++s) {
+            if (gr[s].ncell > sys[s].max_cells) sys[s].max_cells = gr[s].ncell + gr[s].ncell / 4 + 8;
+            sys[s].coff = (int)coff;
+            coff += sys[s].max_cells + 1;
+        }
+        if (coff > (1LL << 30)) throw Fail{ALMA_ERUNTIME, "cell grids too large for one batch"};
+        b.cells = (int)coff;
+        up(&b.sys, sys);
+        dev_alloc(&b.cell_cnt, coff);
+        dev_alloc(&b.cell_start, coff);
+    }
 }
 
 void pt_run(alma_ctx* ctx, const alma_pt_params* p, double* xyz_out, alma_pt_stats* out, int64_t* swaps) {
@@ -1004,34 +1197,51 @@ void pt_run(alma_ctx* ctx, const alma_pt_params* p, double* xyz_out, alma_pt_sta
     }
     ctx->t_pt_ms = 0.0;
     ctx->pt_replays = 0;
+    ctx->pt_regrow = 0;
     if (p->steps > 0) {
-        // one step as a graph (fixed kernel sequence, arguments by value)
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t le = launch_batch_step(b, 0, ctx->stream);
-        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-        HIP_TRY(le);
-        HIP_TRY(ce);
-        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-            hipGraphDestroy(graph);
-            HIP_TRY(ie);
-        }
         if (!ctx->ev0) {
             HIP_TRY(hipEventCreate(&ctx->ev0));
             HIP_TRY(hipEventCreate(&ctx->ev1));
         }
-        hipError_t re = hipEventRecord(ctx->ev0, ctx->stream);
-        for (int k = 0; k < p->steps && re == hipSuccess; ++k) re = hipGraphLaunch(exec, ctx->stream);
-        if (re == hipSuccess) re = hipEventRecord(ctx->ev1, ctx->stream);
-        if (re == hipSuccess) re = hipEventSynchronize(ctx->ev1);
-        float ms = 0.f;
-        if (re == hipSuccess) re = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-        hipGraphExecDestroy(exec);
-        hipGraphDestroy(graph);
-        HIP_TRY(re);
-        ctx->t_pt_ms = ms;
+        // Steps run in blocks of replays of one captured step.  The state a
+        // step mutates (coordinates, gradients, per-system state, slot map,
+        // swap counts) is copied before each block; a block in which some
+        // system overflowed a row capacity or its cell grid is rolled back,
+        // the capacities grow to the counts measured and the block runs
+        // again.  Steps draw from counters of their step number, so the
+        // repeated block is the same and results do not depend on the
+        // capacities.
+        PtSnapshot snap(b);
+        const int block = 8;
+        int done = 0;
+        double ms_total = 0.0;
+        int h_ovf = 0;
+        std::unique_ptr<PtGraph> g(new PtGraph(ctx, b));
+        while (done < p->steps) {
+            const int k = std::min(block, p->steps - done);
+            snap.save(b, ctx->stream);
+            HIP_TRY(hipMemsetAsync(b.flags + 2, 0, sizeof(int), ctx->stream));
+            HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+            for (int r = 0; r < k; ++r) HIP_TRY(hipGraphLaunch(g->exec, ctx->stream));
+            HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(&h_ovf, b.flags + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            ms_total += ms;
+            if (h_ovf) {
+                snap.restore(b, ctx->stream);
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                g.reset();
+                batch_grow(ctx, h_ovf);
+                g.reset(new PtGraph(ctx, b));
+                ctx->pt_regrow += 1;
+                if (ctx->pt_regrow > 64) throw Fail{ALMA_ERUNTIME, "batched systems keep outgrowing their lists"};
+                continue;
+            }
+            done += k;
+        }
+        ctx->t_pt_ms = ms_total;
         ctx->pt_replays = p->steps;
     }
     std::vector<BState> st;
@@ -1373,6 +1583,7 @@ alma_status alma_mc_timing(alma_ctx* ctx, double* out) {
         out[2] = ctx->t_rebuild_ms;
         out[3] = (double)ctx->n_rebuild_launch;
         for (int k = 0; k < 7; ++k) out[4 + k] = (double)ctx->work[k];
+        out[11] = (double)ctx->n_regrow;
     });
 }
 
@@ -1444,6 +1655,7 @@ alma_status alma_pt_timing(alma_ctx* ctx, double* out) {
         out[1] = (double)ctx->pt_replays;
         out[2] = (double)ctx->batch.n_atoms;
         out[3] = (double)ctx->batch.lanes;
+        out[4] = (double)ctx->pt_regrow;
     });
 }
 

@@ -243,7 +243,7 @@ struct Batch {
     uint32_t* nlf = nullptr;   // n_atoms * cap_f, full rows (both directions), scan order
     int* nlf_cnt = nullptr;
     int cap_f = 0;
-    int* flags = nullptr;      // [0] largest row
+    int* flags = nullptr;      // [0] largest half row, [1] largest full row, [2] overflow bits of a block of steps
     // topology, local atom indices, packed over proteins
     int* excl_off = nullptr;
     int* excl_idx = nullptr;
@@ -323,6 +323,7 @@ struct alma_ctx {
     // HIP-event timing of the last MC run: device MC kernel, list rebuilds
     double t_mc_ms = 0.0, t_rebuild_ms = 0.0;
     long long n_mc_launch = 0, n_rebuild_launch = 0;
+    long long n_regrow = 0;   // capacity / cell-grid growths during the last MC run
     long long work[7] = {0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // host copy of the loaded topology (the MALA path builds a batch from it)
@@ -336,6 +337,7 @@ struct alma_ctx {
     bool pt_ready = false;
     double t_pt_ms = 0.0;
     long long pt_replays = 0;
+    long long pt_regrow = 0;   // capacity / cell-grid growths during the last alma_pt_run
     std::vector<int> pt_tr_bond, pt_tr_acc;
     std::vector<double> pt_tr_delta, pt_tr_dE;
     int pt_tr_steps = 0;

@@ -777,6 +777,7 @@ __global__ void __launch_bounds__(kSysBlock) kb_accept(Batch b, int init) {
         double dE = 0.0;
         if (ovf) {
             st.status = (ovf & 2) ? 2 : 1;
+            atomicOr(&b.flags[2], ovf);   // seen by the host after the block of steps
         } else if (init) {
             acc = true;
         } else if (st.status == 0 && valid) {

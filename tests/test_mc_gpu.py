@@ -119,3 +119,23 @@ def test_dropin_generate_ensemble_seeded(ctx):
     orc = O.mc_run(s, S.csr(t), 100, KT, 0.12, seed=22)
     got = np.array([[p.x, p.y, p.z] for p in ens[1]])
     assert np.max(np.abs(got - orc["xyz"])) < 1e-8
+
+
+@pytest.mark.parametrize("caps", ["64,8", "64,0"])
+def test_capacity_growth_keeps_the_trajectory(ctx, monkeypatch, caps):
+    """A list build that overflows its row capacity or the cell grid stops
+    its chain; the capacities grow and the same build runs again.  Starting
+    from tiny capacities must give the trajectory of the default run."""
+    s = S.chain(56, 1)
+    c = S.csr(S.topology(s))
+    _load(ctx, s, c)
+    ref = ctx.mc_run(N.MC_ENSEMBLE, 80, n_chains=3, temperature=KT, seed=11, trace=True)
+    assert ctx.mc_timing()["regrows"] == 0
+    monkeypatch.setenv("ALMA_MC_TEST_CAPS", caps)
+    got = ctx.mc_run(N.MC_ENSEMBLE, 80, n_chains=3, temperature=KT, seed=11, trace=True)
+    assert ctx.mc_timing()["regrows"] >= 1
+    assert np.array_equal(got["trace"]["accepted"], ref["trace"]["accepted"])
+    assert np.array_equal(got["xyz"], ref["xyz"])
+    for a, b in zip(got["stats"], ref["stats"]):
+        assert a["accepted"] == b["accepted"] and a["rebuilds"] == b["rebuilds"], (a, b)
+        assert abs(a["cur_e"] - b["cur_e"]) <= 1e-9 * max(1.0, abs(b["cur_e"]))

@@ -113,3 +113,30 @@ def test_pt_rejects_bad_arguments(ctx):
         ctx.pt_run(3, mala_step=0.0)
     with pytest.raises(ValueError):
         ctx.pt_set(ps[:1], cs[:1], 1, [-1.0], [ps[0]["xyz"]])
+
+
+def test_pt_capacity_growth_keeps_the_run(ctx, monkeypatch):
+    """A block of steps in which a system outgrows its neighbour rows is
+    rolled back and run again with larger rows: starting from tiny rows must
+    give the run of the default capacities."""
+    ps, cs = _proteins()
+    R = 2
+    temps = [KT, 1.6 * KT]
+    xyz = [p["xyz"] for p in ps for _ in range(R)]
+    ctx.pt_set(ps, cs, R, temps, xyz)
+    ref = ctx.pt_run(20, exchange_every=5, mala_step=0.05, seed=9, trace=True)
+    assert ctx.pt_timing()["regrows"] == 0
+    monkeypatch.setenv("ALMA_PT_TEST_CAPS", "64")
+    ctx.pt_set(ps, cs, R, temps, xyz)
+    got = ctx.pt_run(20, exchange_every=5, mala_step=0.05, seed=9, trace=True)
+    assert ctx.pt_timing()["regrows"] >= 1
+    # gradients accumulate through FP64 atomics (order not fixed): coordinates
+    # agree to rounding, decisions exactly
+    for a, b in zip(got["xyz"], ref["xyz"]):
+        assert np.max(np.abs(a - b)) <= 1e-9
+    assert np.array_equal(got["swaps"], ref["swaps"])
+    for key in ("bond", "accepted"):
+        assert np.array_equal(got["trace"][key], ref["trace"][key]), key
+    for a, b in zip(got["stats"], ref["stats"]):
+        assert (a["accepted"], a["moves"], a["slot"], a["status"]) == (b["accepted"], b["moves"], b["slot"], b["status"])
+        assert abs(a["e"] - b["e"]) <= 1e-9 * max(1.0, abs(b["e"]))
