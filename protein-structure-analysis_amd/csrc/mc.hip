@@ -656,6 +656,7 @@ struct Move {
 
 __device__ __forceinline__ void rodrigues_d(const double* p, const double* o, const double* u, double c,
                                             double sn, double* out) {
+#pragma clang fp contract(off)
     double vx = p[0] - o[0], vy = p[1] - o[1], vz = p[2] - o[2];
     double dot = u[0] * vx + u[1] * vy + u[2] * vz;
     double cx = u[1] * vz - u[2] * vy, cy = u[2] * vx - u[0] * vz, cz = u[0] * vy - u[1] * vx;
@@ -666,6 +667,7 @@ __device__ __forceinline__ void rodrigues_d(const double* p, const double* o, co
 }
 
 __device__ bool axis_of(const double* pos, const DevTopo& t, int bond, double* o, double* u) {
+#pragma clang fp contract(off)
     int bi = t.rb_i[bond], bj = t.rb_j[bond];
     for (int d = 0; d < 3; ++d) {
         o[d] = pos[4 * bi + d];
@@ -1168,9 +1170,11 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     const bool gi = group(i) != 0;
                     const double* pin = gi ? posn + 4 * i : pos + 4 * i;
                     const double ri = (double)s.prm[i].y + kProbe;
-                    const double k2 = M_PI * ri;
-                    // the fold at the current coordinates is kept per atom
-                    // (set at the list build, replaced on every accept)
+                    // same expression as the fold of the list build, whose
+                    // value for the current coordinates is kept per atom (set
+                    // at the build, replaced on every accept): an atom whose
+                    // contacts did not change gives exactly its kept value
+                    const double k2 = 2.0 * M_PI * ri * ri / (2.0 * ri);
                     double sa_n = 4.0 * M_PI * ri * ri;
                     int m;
                     const uint32_t* row;
@@ -1228,6 +1232,12 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     }
                 }
                 for (int k = tid; k < T.n_ss; k += kMcBlock) {
+                    // no FMA contraction: with drn == dro (a move that does
+                    // not change the pair) the change must be exactly 0, as
+                    // in the reference, not the rounding error of dro^2 --
+                    // the Metropolis test draws a random number only when
+                    // dE >= 0
+#pragma clang fp contract(off)
                     int i = T.ss_i[k], j = T.ss_j[k];
                     int gi = group(i), gj = group(j);
                     if (!gi && !gj) continue;

@@ -1,0 +1,165 @@
+"""PDB input path (alma.pdb, alma.ff14sb.get_atom_params) against vectors from
+the reference's own parameter module and its compiled physics module
+(tests/golden/make_pdb_golden.py), on three of the reference's data files:
+  * get_atom_params over every (residue, atom) key of the reference tables
+    plus probes (ions, unknown residues, element fallbacks): exact;
+  * hydrogen-parent table (children and their order) and the united-atom
+    charge sums: exact;
+  * per-atom charge / R* / epsilon of each structure, as _parse_pdb assigns
+    them: exact; disulfides detected: equal;
+  * the reader itself against the PDB text: atom counts of the first model,
+    float32 coordinates, residue numbering, CA list;
+  * topology of the drop-in module on the parsed structures: bit-exact;
+  * the FP64 oracle's energy terms against the reference's: 1e-11 relative.
+CPU only."""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+from alma import ff14sb as F
+from alma import pdb
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+STRUCTURES = ("1XQ8", "SM_P01308", "SM_P21543")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(os.path.join(GOLDEN, "pdb_golden.npz"), allow_pickle=False)
+
+
+def _path(name):
+    return os.path.join(GOLDEN, "pdb", name + ".pdb.gz")
+
+
+def test_parameter_table_matches_reference(golden):
+    bad = []
+    for r, a, ref in zip(golden["tab_res"], golden["tab_atom"], golden["tab_params"]):
+        got = F.get_atom_params(str(r), str(a))
+        if tuple(got) != tuple(ref):
+            bad.append((str(r), str(a), got, tuple(ref)))
+    assert not bad, bad[:10]
+    assert len(golden["tab_res"]) > 600
+
+
+def test_hydrogen_parents_match_reference(golden):
+    ref = {(str(r), str(h)): tuple(str(c).split(",")) for r, h, c in
+           zip(golden["hp_res"], golden["hp_heavy"], golden["hp_children"])}
+    assert F.HEAVY_CHILDREN == ref
+    for (r, h), v in zip(ref, golden["hp_all_missing"]):
+        assert F.missing_hydrogen_charge(r, h, set()) == v, (r, h)
+        assert F.missing_hydrogen_charge(r, h, set(ref[(r, h)])) == 0.0
+
+
+def _records(name):
+    """ATOM / HETATM lines of the first model, straight from the text."""
+    out = []
+    with gzip.open(_path(name), "rt") as f:
+        for line in f:
+            if line.startswith("ENDMDL"):
+                break
+            if line.startswith(("ATOM  ", "HETATM")):
+                out.append(line)
+    return out
+
+
+@pytest.mark.parametrize("name", STRUCTURES)
+def test_structure_parameters_match_reference(golden, name):
+    s = pdb.load(_path(name))
+    assert len(s["xyz"]) == int(golden[name + "/n"])
+    ref = golden[name + "/params"]
+    got = np.stack([s["charge"], s["rstar"], s["eps"]], axis=1)
+    assert np.array_equal(got, ref), np.argwhere(got != ref)[:5]
+    assert np.array_equal(s["ss_pairs"], golden[name + "/ss"])
+
+
+@pytest.mark.parametrize("name", STRUCTURES)
+def test_reader_follows_the_text(name):
+    s = pdb.load(_path(name))
+    lines = _records(name)
+    assert len(s["xyz"]) == len(lines)     # no water, nucleotides, altlocs or NaN in these files
+    for k, line in enumerate(lines):
+        want = [float(np.float32(float(line[c:c + 8]))) for c in (30, 38, 46)]
+        assert s["xyz"][k].tolist() == want, k
+        assert s["atomnames"][k] == line[12:16].strip() and s["resnames"][k] == line[17:20].strip(), k
+    std = ~s["het"]
+    assert len(s["ca_indices"]) == len(set(s["res_idx"][std].tolist()))
+    # standard residues numbered 0.. in order of appearance, HETATM from 100000
+    first = np.unique(s["res_idx"][std], return_index=True)[1]
+    assert np.array_equal(np.sort(s["res_idx"][std][np.sort(first)]), np.arange(len(first)))
+    assert np.all(s["res_idx"][s["het"]] >= pdb.HET_RES_BASE)
+
+
+def test_ion_and_disulfides():
+    s = pdb.load(_path("SM_P21543"))
+    ion = np.flatnonzero(s["het"])
+    assert len(ion) == 1 and s["resnames"][ion[0]] == "CA"
+    assert (s["charge"][ion[0]], s["rstar"][ion[0]], s["eps"][ion[0]]) == F.ION_PARAMS["CA"]
+    ss = pdb.load(_path("SM_P01308"))["ss_pairs"]
+    assert len(ss) == 3
+    x = pdb.load(_path("SM_P01308"))["xyz"]
+    for i, j in ss:
+        assert np.sum((x[i] - x[j]) ** 2) < 6.25
+
+
+def test_united_atom_fold_keeps_each_residue_charge():
+    """With hydrogens folded onto their heavy atoms, every complete standard
+    residue carries its template's total charge (hydrogens included), both
+    for a heavy-atom-only model and for an explicit-hydrogen structure
+    (1XQ8), where only absent hydrogens are folded.  A hydrogen not folded
+    would be off by 0.03-0.45 e."""
+    for name in ("SM_P01308", "SM_P21543", "1XQ8"):
+        s = pdb.load(_path(name))
+        res = {}
+        for k, ri in enumerate(s["res_idx"]):
+            if not s["het"][k]:
+                res.setdefault(int(ri), []).append(k)
+        checked = 0
+        for ri, atoms in res.items():
+            rn = s["resnames"][atoms[0]]
+            rec = F.RESIDUES[rn]
+            names = {s["atomnames"][k] for k in atoms}
+            if any(a not in names for a, _, _ in rec if not a.startswith("H")):
+                continue    # incomplete residue
+            q = sum(s["charge"][k] for k in atoms)
+            assert abs(q - sum(c for _, _, c in rec)) < 1e-9, (name, ri, rn, q)
+            checked += 1
+        assert checked > 0.9 * len(res), (name, checked, len(res))
+
+
+@pytest.fixture(scope="module", params=STRUCTURES)
+def parsed(request):
+    import protein_physics as PP
+    s = pdb.load(_path(request.param))
+    t = pdb.topology(s, PP)
+    return request.param, s, t, O.topo_to_csr(t)
+
+
+def test_topology_matches_reference(golden, parsed):
+    name, s, t, c = parsed
+    g = {k.split("/", 1)[1]: golden[k] for k in golden.files if k.startswith(name + "/")}
+    excl = t.excl
+    assert np.array_equal(np.concatenate([[0], np.cumsum([len(v) for v in excl])]), g["t_excl_off"])
+    assert np.array_equal(np.array([j for v in excl for j in v]), g["t_excl_idx"])
+    for key, got in (("t_rb_i", c["rb_i"]), ("t_rb_j", c["rb_j"]), ("t_rb_kind", c["rb_kind"]),
+                     ("t_side_off", c["side_off"]), ("t_dih_a", c["dih_a"]), ("t_dih_d", c["dih_d"]),
+                     ("t_dih_toff", c["dih_toff"]), ("t_dih_n", c["dih_n"])):
+        assert np.array_equal(got, g[key]), (name, key)
+    assert np.array_equal(c["rb_scale"], g["t_rb_scale"])
+    assert np.array_equal(np.array(t.disulfide_pairs).reshape(-1, 2), g["t_ss"])
+    assert np.array_equal(np.array(t.concerted_pairs).reshape(-1, 2), g["t_cp"])
+    assert np.array_equal(np.array(t.concerted_sidechain_pairs).reshape(-1, 2), g["t_csp"])
+    assert np.array_equal(np.array(t.bonds).reshape(-1, 2), g["t_bonds"])
+
+
+def test_oracle_energy_matches_reference(golden, parsed):
+    name, s, t, c = parsed
+    got = O.energy_terms(s, c)
+    ref = golden[name + "/terms"]
+    for k, term in enumerate(O.TERMS):
+        assert abs(got[term] - ref[k]) <= 1e-11 * max(abs(ref[k]), 1e-300) or got[term] == ref[k], \
+            (name, term, got[term], ref[k])
+    assert abs(got["TOTAL"] - float(golden[name + "/potential"])) <= 1e-11 * abs(float(golden[name + "/potential"]))

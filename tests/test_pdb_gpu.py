@@ -1,0 +1,90 @@
+"""Structures read from PDB files (the reference's own data files) through the
+GPU path: per-term energies against the reference's calculate_potential on
+the same particles (tests/golden/pdb_golden.npz), the drop-in module's
+calculate_potential, forces and MC trajectories against the FP64 oracle.
+Tolerances as for the synthetic systems: 1e-5 relative per term; forces
+1e-5 * (|g_i| + rms g); trajectories with the same accept decisions,
+coordinates within 1e-8 A."""
+import os
+
+import numpy as np
+import pytest
+
+from alma import _native as N
+from alma import pdb
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+STRUCTURES = ("1XQ8", "SM_P01308", "SM_P21543")
+KT = 0.0019872041 * 300.0
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(os.path.join(GOLDEN, "pdb_golden.npz"), allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+def _parsed(name):
+    import protein_physics as PP
+    s = pdb.load(os.path.join(GOLDEN, "pdb", name + ".pdb.gz"))
+    t = pdb.topology(s, PP)
+    return s, t, O.topo_to_csr(t)
+
+
+@pytest.mark.parametrize("name", STRUCTURES)
+def test_energy_terms_match_reference(ctx, golden, name):
+    s, t, c = _parsed(name)
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    got = ctx.energy()
+    ref = golden[name + "/terms"]
+    for k, term in enumerate(O.TERMS):
+        if ref[k] == 0.0:
+            assert got[term] == 0.0, (name, term)
+        else:
+            assert abs(got[term] - ref[k]) <= 1e-5 * abs(ref[k]), (name, term, got[term], ref[k])
+
+
+@pytest.mark.parametrize("name", ("1XQ8", "SM_P21543"))
+def test_dropin_calculate_potential(golden, name):
+    import protein_physics as PP
+    s, t, _ = _parsed(name)
+    e = PP.PhysicsEngine().calculate_potential(pdb.particles(s, PP), t)
+    ref = float(golden[name + "/potential"])
+    assert abs(e - ref) <= 1e-5 * abs(ref), (name, e, ref)
+
+
+def test_forces_match_oracle(ctx):
+    s, t, c = _parsed("SM_P01308")
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    e, g, gt = ctx.forces()
+    E, G, GT = O.forces(s, c)
+    assert abs(e - E) <= 1e-5 * abs(E)
+    err = np.linalg.norm(g - G, axis=1)
+    mag = np.linalg.norm(G, axis=1)
+    assert np.all(err <= 1e-5 * (mag + np.sqrt(np.mean(mag ** 2))))
+    assert np.all(np.abs(gt - GT) <= 1e-5 * (np.abs(GT) + np.sqrt(np.mean(GT ** 2))))
+
+
+@pytest.mark.parametrize("name,mode", [("1XQ8", N.MC_ENSEMBLE), ("SM_P01308", N.MC_LANDSCAPE)])
+def test_mc_trajectory_matches_oracle(ctx, name, mode):
+    s, t, c = _parsed(name)
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    steps = 60
+    gpu = ctx.mc_run(mode, steps, n_chains=2, temperature=KT, max_angle=0.12, seed=2024, trace=True)
+    orc = O.mc_run(s, c, steps, KT, 0.12, seed=2024, mode=mode, trace=True, l1=True)
+    gt, ot = gpu["trace"], orc["trace"]
+    assert np.array_equal(gt["kind"], ot["kind"]), name
+    assert np.array_equal(gt["accepted"], ot["accepted"]), name
+    assert np.all(np.abs(gt["dE"] - ot["dE"]) <= 1e-5 * ot["dE_l1"] + 1e-9), name
+    assert np.max(np.abs(gpu["xyz"][0] - orc["xyz"])) < 1e-8, name
+    o2 = O.mc_run(s, c, steps, KT, 0.12, seed=2025, mode=mode)
+    assert np.max(np.abs(gpu["xyz"][1] - o2["xyz"])) < 1e-8, name
