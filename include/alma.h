@@ -87,7 +87,9 @@ enum {
     ALMA_MC_ENSEMBLE = 0,   /* generate_ensemble chains (src/physics_engine.cpp:2273-2543) */
     ALMA_MC_LANDSCAPE = 1,  /* run_landscape_trajectory / _segment moves (:2556-3270)      */
     ALMA_MC_TORSION30 = 2,  /* one uniformly chosen torsion, delta ~ U[-pi/6, pi/6]         */
-    ALMA_MC_MALA = 3        /* gradient-guided torsion proposals (no reference counterpart) */
+    ALMA_MC_MALA = 3,       /* gradient-guided torsion proposals (no reference counterpart) */
+    ALMA_MC_DIAGNOSTIC = 4  /* run_mc_diagnostic moves and records (:3300-3667); through
+                               alma_mc_diagnostic */
 };
 
 typedef struct {
@@ -124,6 +126,20 @@ alma_status alma_mc_run(alma_ctx* ctx, const alma_mc_params* p, double* xyz_out,
  * be NULL. */
 alma_status alma_mc_trace(alma_ctx* ctx, int32_t* kind, int32_t* bond, double* delta,
                           double* dE, int32_t* accepted, double* dE_parts);
+
+/* PhysicsEngine::run_mc_diagnostic (src/physics_engine.cpp:3300-3667,
+ * binding :3873-3884): one chain of n_steps landscape moves from the loaded
+ * system, seeded with mt19937(seed); a degenerate proposal is retried
+ * without counting, a concerted side-chain move counts but is not logged,
+ * every torsion / crankshaft move is logged: move type (0 / 1), |delta|,
+ * accepted, and its energy change split into hard core, the rest of the
+ * pair sum, SASA, dihedral and SS.  Arrays hold n_steps entries; *n_records
+ * receives the number logged.  ALMA_ERUNTIME if 10^6 proposals in a row are
+ * degenerate (the reference would not return). */
+alma_status alma_mc_diagnostic(alma_ctx* ctx, int32_t n_steps, double temperature, double max_angle,
+                               uint32_t seed, int32_t* n_records, int32_t* move_type, double* proposed_delta,
+                               int32_t* accepted, double* d_hardcore, double* d_soft, double* d_sasa,
+                               double* d_dih, double* d_ss);
 
 /* Measurement of the last alma_mc_run (no reference counterpart): out[12] =
  * {device MC kernel ms (HIP events on the context stream), MC launches,

@@ -389,6 +389,10 @@ struct Chain {
     double last_delta = 0, last_dE = 0, last_l1 = 0;
     double last_parts[4] = {0, 0, 0, 0};
     int last_acc = 0;
+    // run_mc_diagnostic: hard-core sub-total of the last cross() and the
+    // cross sums / hard-core sums of the last move's old and new states
+    mutable double cross_hc = 0.0;
+    double last_cross[2] = {0, 0}, last_hc[2] = {0, 0};
 
     Chain(const Sys& sys, const double* x0, const orc_topo* topo, uint32_t seed, double max_angle)
         : s(sys), t(topo), x(x0, x0 + 3 * (size_t)sys.n), a(sys.n), rng(seed), cur_max(max_angle),
@@ -411,6 +415,7 @@ struct Chain {
     // pairs with different group labels (cross_e_concerted :2766-2777).
     double cross(Items* it) const {
         double E = 0.0;
+        cross_hc = 0.0;
         if (it) it->pair.assign(nl.js.size(), 0.0);
         for (int i = 0; i < s.n; ++i)
             for (int k = nl.off[i]; k < nl.off[i + 1]; ++k) {
@@ -418,9 +423,11 @@ struct Chain {
                 if (group[i] == group[j]) continue;
                 if (excluded(t, i, j)) continue;
                 if (dist2(x.data(), i, j) > PAIR_CUT2) continue;
+                double t4[4];
                 double e = pair_terms(&x[3 * i], &x[3 * j], s.q[i], s.q[j], s.rs[i], s.rs[j], s.ep[i],
-                                      s.ep[j], a[i], a[j], nullptr);
+                                      s.ep[j], a[i], a[j], t4);
                 E += e;
+                cross_hc += t4[3];   // pair_e_diag's hard-core share (:2180-2199)
                 if (it) it->pair[k] = e;
             }
         return E;
@@ -507,12 +514,16 @@ struct Chain {
         }
         Items io, in;
         Terms o = terms(want_items ? &io : nullptr);
+        last_cross[0] = o.cross;
+        last_hc[0] = cross_hc;
         if (!move()) {  // degenerate second axis (crankshaft): positions restored
             restore(moved, op, ob);
             return {false, false};
         }
         for (int i : born_set) born_update(i, x.data(), s.rs, nl, a.data());
         Terms nw = terms(want_items ? &in : nullptr);
+        last_cross[1] = nw.cross;
+        last_hc[1] = cross_hc;
         double dE = (nw.cross - o.cross) + (nw.sasa - o.sasa) + (nw.dih - o.dih) + (nw.ss - o.ss);
         last_dE = dE;
         last_parts[0] = nw.cross - o.cross;
@@ -606,6 +617,8 @@ struct Chain {
         }
         Items io, in;
         Terms o = terms(want_items ? &io : nullptr);
+        last_cross[0] = o.cross;
+        last_hc[0] = cross_hc;
         double delta = rng.uniform_real(-cur_max, cur_max);
         last_delta = delta;
         double c = std::cos(delta), sn = std::sin(delta);
@@ -622,6 +635,8 @@ struct Chain {
         for (int i : ss) rodrigues(&x[3 * i], o2, u2, c, -sn);
         for (int i : ps) born_update(i, x.data(), s.rs, nl, a.data());
         Terms nw = terms(want_items ? &in : nullptr);
+        last_cross[1] = nw.cross;
+        last_hc[1] = cross_hc;
         double dE = (nw.cross - o.cross) + (nw.sasa - o.sasa) + (nw.dih - o.dih) + (nw.ss - o.ss);
         last_dE = dE;
         last_parts[0] = nw.cross - o.cross;
@@ -1102,6 +1117,64 @@ int orc_mc_run(int n, double* xyz, const double* q, const double* rstar, const d
     for (int64_t k = 0; k < p->skip; ++k) ch.rng();
     ch.rng.used = 0;
     ch.want_items = tr && tr->dE_l1;
+    if (p->mode == ORC_MODE_DIAGNOSTIC) {
+        // run_mc_diagnostic, src/physics_engine.cpp:3300-3667: landscape moves;
+        // a degenerate proposal is retried without counting; a concerted
+        // side-chain move counts as a step but is neither logged nor tuned;
+        // torsion / crankshaft moves are logged and tune the step size.
+        int rec = 0, stuck = 0;
+        const double T = p->temperature;
+        for (int k = 0; k < p->steps; ++k) {
+            ch.rebuild_if_needed();
+            if (++stuck > 1000000) return -2;   // only degenerate proposals: the reference never returns
+            if (topo->n_csp > 0 && ch.rng.canonical() < 0.15) {
+                auto r = ch.concerted(T);
+                if (!r.second) {
+                    --k;
+                    continue;
+                }
+                stuck = 0;
+                ++ch.moves;
+                if (r.first) ++ch.accepted;
+                continue;
+            }
+            bool crank = topo->n_cp > 0 && ch.rng.canonical() < 0.25;
+            auto r = crank ? ch.crankshaft(T) : ch.torsion(T, false);
+            if (!r.second) {
+                --k;
+                continue;
+            }
+            stuck = 0;
+            ++ch.moves;
+            if (r.first) ++ch.accepted;
+            ch.tune(r.first);
+            if (tr) {
+                if (tr->kind) tr->kind[rec] = ch.last_kind;
+                if (tr->bond) tr->bond[rec] = ch.last_bond;
+                if (tr->delta) tr->delta[rec] = std::fabs(ch.last_delta);
+                if (tr->dE) tr->dE[rec] = ch.last_dE;
+                if (tr->dE_l1) tr->dE_l1[rec] = ch.last_l1;
+                if (tr->accepted) tr->accepted[rec] = ch.last_acc;
+                if (tr->parts)
+                    for (int q = 0; q < 4; ++q) tr->parts[4 * rec + q] = ch.last_parts[q];
+                if (tr->hardcore) tr->hardcore[rec] = ch.last_hc[1] - ch.last_hc[0];
+                if (tr->soft)
+                    tr->soft[rec] = (ch.last_cross[1] - ch.last_hc[1]) - (ch.last_cross[0] - ch.last_hc[0]);
+            }
+            ++rec;
+        }
+        if (tr && tr->records) *tr->records = rec;
+        std::copy(ch.x.begin(), ch.x.end(), xyz);
+        if (st) {
+            st->cur_e = ch.curE;
+            st->cur_max = ch.cur_max;
+            st->accepted = ch.accepted;
+            st->moves = ch.moves;
+            st->rebuilds = ch.rebuilds;
+            st->words = ch.rng.used;
+        }
+        return 0;
+    }
     int snap = 0;
     for (int k = 0; k < p->steps; ++k) {
         int64_t before = ch.rng.used;

@@ -59,7 +59,9 @@ void orc_uniform_real(uint32_t seed, double a, double b, int64_t n, double* out)
 /* ---- MC ---- */
 enum { ORC_MODE_ENSEMBLE = 0,   /* generate_ensemble chain semantics      */
        ORC_MODE_LANDSCAPE = 1,  /* run_landscape_* semantics (concerted)   */
-       ORC_MODE_TORSION30 = 2   /* single torsion, delta ~ U[-pi/6, pi/6]  */ };
+       ORC_MODE_TORSION30 = 2,  /* single torsion, delta ~ U[-pi/6, pi/6]  */
+       ORC_MODE_DIAGNOSTIC = 4  /* run_mc_diagnostic: records per torsion /
+                                   crankshaft move (see orc_mc_trace)       */ };
 
 typedef struct {
     int mode;
@@ -91,6 +93,13 @@ typedef struct {
     int* accepted;
     int64_t* words_before;  /* RNG cursor before the step                      */
     double* parts;          /* 4 per step: cross pairs, SASA, dihedral, SS     */
+    /* ORC_MODE_DIAGNOSTIC: the arrays above hold one record per logged
+     * (torsion / crankshaft) move instead of one per step, delta as
+     * |delta|; hardcore / soft: the hard-core change and the pair change
+     * without it; records: number of logged moves. */
+    double* hardcore;
+    double* soft;
+    int* records;
 } orc_mc_trace;
 
 /* Run one chain.  xyz is updated in place.  snap_xyz (n_snap*3n) and

@@ -14,6 +14,7 @@ _LIB = None
 
 TERMS = ("DH", "GB", "LJ", "HC", "SASA", "DIH", "SS", "TOTAL")
 MODE_ENSEMBLE, MODE_LANDSCAPE, MODE_TORSION30 = 0, 1, 2
+MODE_DIAGNOSTIC = 4
 
 _I = C.POINTER(C.c_int)
 _D = C.POINTER(C.c_double)
@@ -43,7 +44,8 @@ class McStats(C.Structure):
 
 class McTrace(C.Structure):
     _fields_ = [("kind", _I), ("bond", _I), ("delta", _D), ("dE", _D), ("dE_l1", _D),
-                ("accepted", _I), ("words_before", C.POINTER(C.c_int64)), ("parts", _D)]
+                ("accepted", _I), ("words_before", C.POINTER(C.c_int64)), ("parts", _D),
+                ("hardcore", _D), ("soft", _D), ("records", _I)]
 
 
 def lib():
@@ -222,6 +224,41 @@ def mc_run(sysd, topo_csr, steps, temperature=0.6, max_angle=0.12, seed=5489,
         res["snapshots"] = snap_x[:nsnap]
         res["energies"] = snap_e[:nsnap]
     return res
+
+
+def mc_diagnostic(sysd, topo_csr, n_steps, temperature=0.6, max_angle=0.12, seed=5489, l1=False):
+    """run_mc_diagnostic (src/physics_engine.cpp:3300-3667) seeded with
+    mt19937(seed): one record per logged torsion / crankshaft move --
+    move_type, |delta|, accepted, d_hardcore, d_nonbonded_soft, d_sasa,
+    d_dih, d_ss (the reference's tuple), plus dE, the per-item error scale
+    (l1) and the final coordinates."""
+    xyz = _f64(sysd["xyz"]).copy()
+    n = len(xyz)
+    th = TopoHandle(topo_csr, n)
+    p = McParams(MODE_DIAGNOSTIC, n_steps, 0, temperature, max_angle, seed, 0)
+    st = McStats()
+    a = {"kind": np.zeros(n_steps, np.int32), "bond": np.zeros(n_steps, np.int32),
+         "delta": np.zeros(n_steps), "dE": np.zeros(n_steps), "dE_l1": np.zeros(n_steps) if l1 else None,
+         "accepted": np.zeros(n_steps, np.int32), "parts": np.zeros((n_steps, 4)),
+         "hardcore": np.zeros(n_steps), "soft": np.zeros(n_steps)}
+    rec = np.zeros(1, np.int32)
+    tr = McTrace(_p(a["kind"], C.c_int), _p(a["bond"], C.c_int), _p(a["delta"], C.c_double),
+                 _p(a["dE"], C.c_double), _p(a["dE_l1"], C.c_double), _p(a["accepted"], C.c_int), None,
+                 _p(a["parts"], C.c_double), _p(a["hardcore"], C.c_double), _p(a["soft"], C.c_double),
+                 _p(rec, C.c_int))
+    rc = lib().orc_mc_run(n, _p(xyz, C.c_double), _p(_f64(sysd["charge"]), C.c_double),
+                          _p(_f64(sysd["rstar"]), C.c_double), _p(_f64(sysd["eps"]), C.c_double),
+                          th.ptr(), C.byref(p), C.byref(st), C.byref(tr), None, None)
+    if rc != 0:
+        raise ValueError("orc_mc_run rejected its arguments (%d)" % rc)
+    m = int(rec[0])
+    out = {"move_type": a["kind"][:m], "proposed_delta": a["delta"][:m], "accepted": a["accepted"][:m],
+           "d_hardcore": a["hardcore"][:m], "d_nonbonded_soft": a["soft"][:m], "d_sasa": a["parts"][:m, 1],
+           "d_dih": a["parts"][:m, 2], "d_ss": a["parts"][:m, 3], "dE": a["dE"][:m], "xyz": xyz,
+           "stats": {f: getattr(st, f) for f, _ in McStats._fields_}}
+    if l1:
+        out["dE_l1"] = a["dE_l1"][:m]
+    return out
 
 
 def mc_multi(sysd, topo_csr, nchains, steps, temperature=0.6, max_angle=0.12, seed=5489,

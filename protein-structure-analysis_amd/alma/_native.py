@@ -96,6 +96,8 @@ def lib():
         L.alma_mc_trace.argtypes = [P, _I, _I, _D, _D, _I, _D]
         L.alma_forces.argtypes = [P, _D, _D, _D]
         L.alma_mc_timing.argtypes = [P, _D]
+        L.alma_mc_diagnostic.argtypes = [P, C.c_int32, C.c_double, C.c_double, C.c_uint32, _I, _I, _D, _I, _D,
+                                         _D, _D, _D, _D]
         L.alma_pt_set.argtypes = [P, C.POINTER(PtSystems), _D]
         L.alma_pt_run.argtypes = [P, C.POINTER(PtParams), _D, C.POINTER(PtStats), C.POINTER(C.c_int64)]
         L.alma_pt_trace.argtypes = [P, _I, _D, _D, _I]
@@ -107,7 +109,7 @@ def lib():
 
 EXPORTED = ("alma_create", "alma_destroy", "alma_last_error", "alma_device_name",
             "alma_set_system", "alma_set_positions", "alma_energy", "alma_nlist",
-            "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_mc_timing", "alma_forces",
+            "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_mc_timing", "alma_mc_diagnostic", "alma_forces",
             "alma_pt_set", "alma_pt_run", "alma_pt_trace", "alma_pt_energy", "alma_pt_timing")
 
 
@@ -248,6 +250,20 @@ class Context:
                                             _p(tr["parts"], C.c_double)))
             out["trace"] = tr
         return out
+
+    def mc_diagnostic(self, n_steps, temperature=0.6, max_angle=0.12, seed=5489) -> dict:
+        """run_mc_diagnostic records of one seeded chain of the loaded system."""
+        a = {"move_type": np.zeros(n_steps, np.int32), "proposed_delta": np.zeros(n_steps),
+             "accepted": np.zeros(n_steps, np.int32), "d_hardcore": np.zeros(n_steps),
+             "d_nonbonded_soft": np.zeros(n_steps), "d_sasa": np.zeros(n_steps), "d_dih": np.zeros(n_steps),
+             "d_ss": np.zeros(n_steps)}
+        mc = np.zeros(1, np.int32)
+        self._check(lib().alma_mc_diagnostic(
+            self._h, n_steps, C.c_double(temperature), C.c_double(max_angle), C.c_uint32(seed), _p(mc, C.c_int32),
+            _p(a["move_type"], C.c_int32), _p(a["proposed_delta"], C.c_double), _p(a["accepted"], C.c_int32),
+            _p(a["d_hardcore"], C.c_double), _p(a["d_nonbonded_soft"], C.c_double), _p(a["d_sasa"], C.c_double),
+            _p(a["d_dih"], C.c_double), _p(a["d_ss"], C.c_double)))
+        return {k: v[:int(mc[0])] for k, v in a.items()}
 
     def mc_timing(self) -> dict:
         out = np.zeros(12)

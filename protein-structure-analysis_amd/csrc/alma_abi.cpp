@@ -368,6 +368,7 @@ void free_chains(ChainBatch& b) {
     dev_free(b.tr_dE);
     dev_free(b.tr_acc);
     dev_free(b.tr_parts);
+    dev_free(b.tr_hc);
     b = ChainBatch{};
 }
 
@@ -425,6 +426,7 @@ void alloc_chains(ChainBatch& b, const DevSys& s, int nch, int trace_steps) {
         dev_alloc(&b.tr_dE, trace_steps);
         dev_alloc(&b.tr_acc, trace_steps);
         dev_alloc(&b.tr_parts, 4 * (size_t)trace_steps);
+        dev_alloc(&b.tr_hc, trace_steps);
     }
 }
 
@@ -508,8 +510,10 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     if (s.n == 0) throw Fail{ALMA_EINVAL, "initial_state empty"};
     if (!s.has_topo || s.topo.n_rot <= 0) throw Fail{ALMA_EINVAL, "topology has no rotatable bonds"};
     if (p.n_chains <= 0 || p.steps <= 0) throw Fail{ALMA_EINVAL, "ncand/steps must be positive"};
-    if (p.mode < ALMA_MC_ENSEMBLE || p.mode > ALMA_MC_TORSION30)
+    if (p.mode < ALMA_MC_ENSEMBLE || (p.mode > ALMA_MC_TORSION30 && p.mode != ALMA_MC_DIAGNOSTIC))
         throw Fail{ALMA_EINVAL, "unsupported MC mode"};
+    if (p.mode == ALMA_MC_DIAGNOSTIC && (!p.trace || p.snapshot_every))
+        throw Fail{ALMA_EINVAL, "diagnostic runs record a trace and take no snapshots"};
     if (p.snapshot_every < 0) throw Fail{ALMA_EINVAL, "snapshot interval must be >= 0"};
     if (!(p.temperature > 0.0)) throw Fail{ALMA_EINVAL, "temperature must be positive"};
     const int nch = p.n_chains, n = s.n;
@@ -599,6 +603,7 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
         for (int c = 0; c < nch; ++c) {
             ChainState& x = st[c];
             if (x.status == CS_ERROR) throw Fail{ALMA_ERUNTIME, "MC chain ran out of random words"};
+            if (x.status == CS_STUCK) throw Fail{ALMA_ERUNTIME, "MC proposals are all degenerate"};
             if (x.status == CS_WORDS) {
                 // keep the unconsumed tail, append fresh words
                 int tail = x.wavail - x.cursor;
@@ -645,7 +650,7 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
         if (dirty) HIP_TRY(hipMemcpy(b.st, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpyAsync(b.list, run.data(), sizeof(int) * run.size(), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        HIP_TRY(launch_chain_mc(b, s, (int)run.size(), ctx->stream));
+        HIP_TRY(launch_chain_mc(b, s, (int)run.size(), p.mode == ALMA_MC_DIAGNOSTIC, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipEventSynchronize(ctx->ev1));
         {
@@ -724,6 +729,9 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
         HIP_TRY(hipMemcpy(ctx->tr_delta.data(), b.tr_delta, sizeof(double) * trace_steps, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(ctx->tr_dE.data(), b.tr_dE, sizeof(double) * trace_steps, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(ctx->tr_acc.data(), b.tr_acc, sizeof(int) * trace_steps, hipMemcpyDeviceToHost));
+        ctx->tr_hc.resize(trace_steps);
+        HIP_TRY(hipMemcpy(ctx->tr_hc.data(), b.tr_hc, sizeof(double) * trace_steps, hipMemcpyDeviceToHost));
+        ctx->tr_records = p.mode == ALMA_MC_DIAGNOSTIC ? st[0].n_logged : trace_steps;
     }
     // device coordinates of the system back to the loaded state
     upload_positions(ctx);
@@ -1572,6 +1580,42 @@ alma_status alma_mc_trace(alma_ctx* ctx, int32_t* kind, int32_t* bond, double* d
         if (dE) std::copy(ctx->tr_dE.begin(), ctx->tr_dE.end(), dE);
         if (accepted) std::copy(ctx->tr_acc.begin(), ctx->tr_acc.end(), accepted);
         if (dE_parts) std::copy(ctx->tr_parts.begin(), ctx->tr_parts.end(), dE_parts);
+    });
+}
+
+alma_status alma_mc_diagnostic(alma_ctx* ctx, int32_t n_steps, double temperature, double max_angle,
+                               uint32_t seed, int32_t* n_records, int32_t* move_type, double* proposed_delta,
+                               int32_t* accepted, double* d_hardcore, double* d_soft, double* d_sasa,
+                               double* d_dih, double* d_ss) {
+    return guarded(ctx, [&] {
+        if (n_steps <= 0) throw Fail{ALMA_EINVAL, "n_steps must be positive"};
+        if (!n_records) throw Fail{ALMA_EINVAL, "record count missing"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        alma_mc_params p{};
+        p.mode = ALMA_MC_DIAGNOSTIC;
+        p.n_chains = 1;
+        p.steps = n_steps;
+        p.temperature = temperature;
+        p.max_angle = max_angle;
+        p.seed = seed;
+        p.trace = 1;
+        p.mala_step = 0.02;
+        std::vector<double> xyz(3 * (size_t)ctx->sys.n);
+        alma_mc_stats st{};
+        run_mc(ctx, p, xyz.data(), &st, nullptr, nullptr);
+        const int m = ctx->tr_records;
+        *n_records = m;
+        for (int k = 0; k < m; ++k) {
+            const double hc = ctx->tr_hc[k];
+            if (move_type) move_type[k] = ctx->tr_kind[k];
+            if (proposed_delta) proposed_delta[k] = ctx->tr_delta[k];
+            if (accepted) accepted[k] = ctx->tr_acc[k];
+            if (d_hardcore) d_hardcore[k] = hc;
+            if (d_soft) d_soft[k] = ctx->tr_parts[4 * k] - hc;
+            if (d_sasa) d_sasa[k] = ctx->tr_parts[4 * k + 1];
+            if (d_dih) d_dih[k] = ctx->tr_parts[4 * k + 2];
+            if (d_ss) d_ss[k] = ctx->tr_parts[4 * k + 3];
+        }
     });
 }
 

@@ -100,7 +100,8 @@ struct DevSys {
 
 // ---- device MC (mc.hip) ----
 // Chain status codes written by the MC kernel.
-enum { CS_RUN = 0, CS_REBUILD = 1, CS_TARGET = 2, CS_WORDS = 3, CS_ERROR = 4, CS_GRID = 5, CS_CAPACITY = 6 };
+enum { CS_RUN = 0, CS_REBUILD = 1, CS_TARGET = 2, CS_WORDS = 3, CS_ERROR = 4, CS_GRID = 5, CS_CAPACITY = 6,
+       CS_STUCK = 7 };
 
 // State of one MC chain (device resident).
 struct ChainState {
@@ -127,6 +128,8 @@ struct ChainState {
     // boundary dihedral records
     long long n_cross, n_pair_eval, n_sasa, n_moved, n_dih;
     long long n_build_pairs, n_build_sasa;   // list builds: half-row pairs (Born), SASA-row entries folded
+    int n_logged;        // diagnostic mode: moves recorded
+    int stuck;           // diagnostic mode: degenerate proposals in a row
 };
 
 // Batch of chains of one system (same topology and parameters, own state).
@@ -174,6 +177,7 @@ struct ChainBatch {
     double* tr_delta = nullptr;
     double* tr_dE = nullptr;
     double* tr_parts = nullptr;   // 4 per step: pairs, SASA, dihedral, SS
+    double* tr_hc = nullptr;      // diagnostic mode: hard-core share of the pair change
     int* tr_acc = nullptr;
 };
 
@@ -283,7 +287,7 @@ hipError_t launch_batch_step(const Batch& b, int init, hipStream_t st);
 // evaluation kernels only, pair kernel variant chosen by lanes / block
 hipError_t launch_batch_pair(const Batch& b, int lanes, int block, hipStream_t st);
 
-hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int nlist, hipStream_t st);
+hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int nlist, bool diag, hipStream_t st);
 // curE of every chain: total energy at its last list build + accepted dE since
 hipError_t launch_chain_energy(ChainBatch& b, const DevSys& s, hipStream_t st);
 
@@ -346,5 +350,6 @@ struct alma_ctx {
     int f_n = -1;
     // MC trace of the last run
     std::vector<int> tr_kind, tr_bond, tr_acc;
-    std::vector<double> tr_delta, tr_dE, tr_parts;
+    std::vector<double> tr_delta, tr_dE, tr_parts, tr_hc;
+    int tr_records = 0;   // diagnostic mode: records logged by chain 0
 };

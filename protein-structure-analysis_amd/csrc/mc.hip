@@ -594,13 +594,20 @@ struct Words {
         }
         return (int)(prod >> 32);
     }
+    // no FMA contraction in the two below: the draws equal the host
+    // library's bit for bit (a contracted a + (b - a) u differs in the last
+    // bit)
     __device__ double canonical() {        // generate_canonical<double, 53>
+#pragma clang fp contract(off)
         double sum = (double)next();
         sum += (double)next() * 4294967296.0;
         double r = sum / 18446744073709551616.0;
         return r >= 1.0 ? 0x1.fffffffffffffp-1 : r;
     }
-    __device__ double uniform_real(double a, double bb) { return canonical() * (bb - a) + a; }
+    __device__ double uniform_real(double a, double bb) {
+#pragma clang fp contract(off)
+        return canonical() * (bb - a) + a;
+    }
 };
 
 struct Runs {
@@ -651,8 +658,20 @@ struct Move {
     double o2[3], u2[3], cs2, sn2;
     double delta, dE;
     double part[4];
+    double hc;   // diagnostic mode: hard-core share of part[0]
     int accept, stop, rebuild, words_at;
 };
+
+// Hard-core share of pair_e_diag (src/physics_engine.cpp:2180-2199), FP64
+__device__ __forceinline__ double hard_core(double r2, double Ri, double Rj) {
+    const double sig = Ri + Rj;
+    const double rc = sig * kHardFrac;
+    if (!(r2 < rc * rc)) return 0.0;
+    const double c2 = rc * rc / r2;
+    const double c6 = c2 * c2 * c2;
+    const double x = c6 * c6 - 1.0;
+    return fmin(kHardScale * x * x, kHardCap);
+}
 
 __device__ __forceinline__ void rodrigues_d(const double* p, const double* o, const double* u, double c,
                                             double sn, double* out) {
@@ -694,6 +713,9 @@ __device__ __forceinline__ int lower_bound_row(const uint32_t* row, int lo, int 
 
 // One workgroup = one chain; runs until its step target, a needed list
 // rebuild, or the end of its RNG words.
+// kDiag: run_mc_diagnostic records (a separate instantiation, so that the
+// other modes carry none of its code or registers)
+template <bool kDiag>
 __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     const int c = b.list[blockIdx.x];
     const int n = b.n;
@@ -724,7 +746,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
     const bool trace = b.trace_cap > 0 && c == 0;
 
     __shared__ Move mv;
-    __shared__ double red[4 * kMcWaves];
+    __shared__ double red[5 * kMcWaves];
     __shared__ int far_move;
     __shared__ int2 lruns[2][kLdsRuns];
     __shared__ int nseg, naff;
@@ -789,6 +811,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             mv.dE = 0.0;
             mv.delta = 0.0;
             mv.part[0] = mv.part[1] = mv.part[2] = mv.part[3] = 0.0;
+            mv.hc = 0.0;
             mv.kind = -1;
             mv.bond = mv.bond2 = -1;
             if (cs->steps_done >= cs->steps_target) {
@@ -802,7 +825,7 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 mv.words_at = W.cur;
                 int mode = cs->mode;
                 int kind = 0;
-                if (mode == ALMA_MC_LANDSCAPE) {
+                if (mode == ALMA_MC_LANDSCAPE || mode == ALMA_MC_DIAGNOSTIC) {
                     bool conc = T.n_csp > 0 && W.canonical() < 0.15;
                     bool crank = !conc && T.n_cp > 0 && W.canonical() < 0.25;
                     kind = conc ? 2 : crank ? 1 : 0;
@@ -1154,6 +1177,23 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 else if (in_n)
                     acc += pair_energy(r2n, ni, nj, Ri, Rj, ei, ej);
             });
+            // diagnostic mode: the hard-core share of the pair change, for
+            // cross pairs inside the cutoff in each geometry (cross_e_diag,
+            // src/physics_engine.cpp:3344-3358)
+            double acc_hc = 0.0;
+            if (kDiag) {
+                for_cross([&](int i, int gi, uint32_t e, int gj, bool is_pair) {
+                    if (!is_pair || (e & kExclBit)) return;
+                    const int j = (int)(e & kIdxMask);
+                    const double* pio = pos + 4 * i;
+                    const double* pjo = pos + 4 * j;
+                    const double r2o = dist2_exact(pio, pjo);
+                    const double r2n = dist2_exact(gi ? posn + 4 * i : pio, gj ? posn + 4 * j : pjo);
+                    const double Ri = s.rstar[i], Rj = s.rstar[j];
+                    if (r2o <= kPairCut2) acc_hc -= hard_core(r2o, Ri, Rj);
+                    if (r2n <= kPairCut2) acc_hc += hard_core(r2n, Ri, Rj);
+                });
+            }
             tick(8);
             // SASA: compact the marked atoms, then one wavefront per atom
             // gathers 64 row entries at a time, ballots the contacts (old and
@@ -1252,17 +1292,22 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             acc_sa = wave_sum(acc_sa);
             acc_dh = wave_sum(acc_dh);
             acc_ss = wave_sum(acc_ss);
+            acc_hc = wave_sum(acc_hc);
             if (lane == 0) {
                 red[wave] = acc;
                 red[kMcWaves + wave] = acc_sa;
                 red[2 * kMcWaves + wave] = acc_dh;
                 red[3 * kMcWaves + wave] = acc_ss;
+                red[4 * kMcWaves + wave] = acc_hc;
             }
             __syncthreads();
             if (tid == 0) {
                 double part[4] = {0.0, 0.0, 0.0, 0.0};
                 for (int q = 0; q < 4; ++q)
                     for (int w = 0; w < kMcWaves; ++w) part[q] += red[q * kMcWaves + w];
+                double hcs = 0.0;
+                for (int w = 0; w < kMcWaves; ++w) hcs += red[4 * kMcWaves + w];
+                mv.hc = hcs;
                 double dE = part[0] + part[1] + part[2] + part[3];
                 for (int q = 0; q < 4; ++q) mv.part[q] = part[q];
                 mv.dE = dE;
@@ -1315,8 +1360,24 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
         // ---- bookkeeping (tid 0)
         if (tid == 0) {
             int step = cs->steps_done;
+            constexpr bool diag = kDiag;
             if (mv.kind >= 0) cs->steps_by_kind[mv.kind] += 1;
-            if (trace && step < b.trace_cap) {
+            if (diag) {
+                // run_mc_diagnostic: torsion / crankshaft moves are logged
+                const int rec = cs->n_logged;
+                if (mv.valid && mv.kind != 2) {
+                    if (trace && rec < b.trace_cap) {
+                        b.tr_kind[rec] = mv.kind;
+                        b.tr_bond[rec] = mv.kind == 1 ? -1 : mv.bond;
+                        b.tr_delta[rec] = fabs(mv.delta);
+                        b.tr_dE[rec] = mv.dE;
+                        for (int q = 0; q < 4; ++q) b.tr_parts[4 * rec + q] = mv.part[q];
+                        b.tr_hc[rec] = mv.hc;
+                        b.tr_acc[rec] = mv.accept;
+                    }
+                    cs->n_logged = rec + 1;
+                }
+            } else if (trace && step < b.trace_cap) {
                 b.tr_kind[step] = mv.kind;
                 b.tr_bond[step] = mv.kind == 1 || mv.kind == 2 ? -1 : mv.bond;
                 b.tr_delta[step] = mv.delta;
@@ -1327,7 +1388,8 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
             if (mv.valid) {
                 cs->moves += 1;
                 if (mv.accept) cs->accepted += 1;
-                if (cs->mode != ALMA_MC_TORSION30) {
+                // the diagnostic's concerted move does not tune the step
+                if (cs->mode != ALMA_MC_TORSION30 && !(diag && mv.kind == 2)) {
                     // adaptive step, src/physics_engine.cpp:2519-2527
                     cs->acc_win += mv.accept;
                     if (++cs->tot_win == 200) {
@@ -1340,7 +1402,16 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     }
                 }
             }
-            cs->steps_done = step + 1;
+            if (diag && !mv.valid) {
+                // degenerate proposal: retried without counting
+                if (++cs->stuck > 1000000) {
+                    cs->status = CS_STUCK;
+                    mv.stop = 1;
+                }
+            } else {
+                cs->steps_done = step + 1;
+                cs->stuck = 0;
+            }
             cs->cursor = W.cur;
             if (cs->status == CS_ERROR) mv.stop = 1;
             if (mv.rebuild) pending = 2;
@@ -1467,7 +1538,7 @@ hipError_t launch_chain_energy(ChainBatch& b, const DevSys& s, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int L, hipStream_t st) {
+hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int L, bool diag, hipStream_t st) {
     if (L <= 0) return hipSuccess;
     auto stage_bytes = [](int n) {
         return (size_t)n * (sizeof(float4) + sizeof(int)) + (size_t)2 * kMcWaves * ((n + 31) / 32) * sizeof(uint32_t);
@@ -1478,11 +1549,16 @@ hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int L, hipStream_t st
     static bool attr_set = false;
     if (!attr_set) {
         size_t cap = std::max(stage_bytes(kLdsRebuild), (size_t)kLdsAtoms * (sizeof(double) + sizeof(int)));
-        hipError_t e = hipFuncSetAttribute((const void*)k_mc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
-        if (e != hipSuccess) return e;
+        for (const void* f : {(const void*)k_mc<false>, (const void*)k_mc<true>}) {
+            hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
+            if (e != hipSuccess) return e;
+        }
         attr_set = true;
     }
-    k_mc<<<L, kMcBlock, lds, st>>>(b, s);
+    if (diag)
+        k_mc<true><<<L, kMcBlock, lds, st>>>(b, s);
+    else
+        k_mc<false><<<L, kMcBlock, lds, st>>>(b, s);
     return hipGetLastError();
 }
 

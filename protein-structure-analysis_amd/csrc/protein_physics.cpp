@@ -473,6 +473,31 @@ PYBIND11_MODULE(ALMA_PY_MODULE, m) {
             py::arg("steps_per_snapshot"), py::arg("temperature") = 0.6, py::arg("max_angle") = 0.12,
             py::arg("seed") = py::none())
         .def(
+            "run_mc_diagnostic",
+            // src/physics_engine.cpp:3873-3884: tuple of 8 lists, one entry per
+            // logged torsion / crankshaft move
+            [](Engine& self, const Parts& init, py::object topo, int n_steps, double T, double max_angle,
+               py::object seed) {
+                Csr c = csr_from_object(topo);
+                check_mc_args(init, c, n_steps, 1, "n_steps must be positive");
+                uint32_t s = draw_seed(seed);
+                std::vector<int32_t> mt(n_steps), acc(n_steps);
+                std::vector<double> dl(n_steps), hc(n_steps), soft(n_steps), sa(n_steps), dh(n_steps), ss(n_steps);
+                int32_t m = 0;
+                {
+                    py::gil_scoped_release rel;
+                    self.load(init, &c);
+                    self.check(alma_mc_diagnostic(self.ctx(), n_steps, T, max_angle, s, &m, mt.data(), dl.data(),
+                                                  acc.data(), hc.data(), soft.data(), sa.data(), dh.data(),
+                                                  ss.data()));
+                }
+                for (auto* v : {&mt, &acc}) v->resize(m);
+                for (auto* v : {&dl, &hc, &soft, &sa, &dh, &ss}) v->resize(m);
+                return py::make_tuple(mt, dl, acc, hc, soft, sa, dh, ss);
+            },
+            py::arg("initial_state"), py::arg("topology"), py::arg("n_steps"), py::arg("temperature") = 0.6,
+            py::arg("max_angle") = 0.12, py::arg("seed") = py::none())
+        .def(
             "lowest_energy_structure",
             [](Engine& self, const std::vector<Parts>& ens) {
                 // calculate_potential without topology, as the reference does

@@ -11,8 +11,9 @@ Sources, all in this container where the reference is present:
     list of this package's reader;
   * the reference module built from its own sources (oracle/_ref/alma_ref,
     see make_golden.py): BondTopology (build, concerted side-chain pairs,
-    disulfides in _parse_pdb's order) and the per-term energies / total of
-    calculate_potential for those particles.
+    disulfides in _parse_pdb's order), the per-term energies / total of
+    calculate_potential for those particles, and seeded run_mc_diagnostic
+    records on two of them.
 
 The structures are the reference's own data files (data/1XQ8.pdb: NMR with
 explicit hydrogens; data/SM_P01308.pdb: three disulfides; data/SM_P21543.pdb:
@@ -36,6 +37,10 @@ import make_golden as G  # noqa: E402
 from alma import pdb  # noqa: E402
 
 STRUCTURES = ["1XQ8", "SM_P01308", "SM_P21543"]
+KT = 0.0019872041 * 300.0
+DIAG = {"1XQ8": (120, 31), "SM_P01308": (200, 57)}   # run_mc_diagnostic: steps, seed
+DIAG_KEYS = ("move_type", "proposed_delta", "accepted", "d_hardcore", "d_nonbonded_soft", "d_sasa", "d_dih",
+             "d_ss")
 PROBES = [("XYZ", n) for n in ("CL1", "BR", "1HB", "OXT", "P1", "FE", "ZN1", "SE", "CA", "C7", "N3", "O1P",
                                "S1", "H12", "F2", "I1", "MG", "NA", "K", "MN1", "Q9", "2HG1")]
 PROBES += [(r, "CA") for r in ("MG", "CA", "ZN", "FE", "FE2", "FE3", "MN", "NI", "CU", "CO", "NA", "CL", "K", "MO")]
@@ -84,6 +89,15 @@ def main():
             out[name + "/" + k] = v
         out[name + "/terms"] = np.array(R.terms(ps, t))
         out[name + "/potential"] = np.array(R.PhysicsEngine().calculate_potential(ps, t))
+        if name in DIAG:
+            # run_mc_diagnostic with the member generator seeded
+            eng = R.PhysicsEngine()
+            R.seed(eng, DIAG[name][1])
+            rec = eng.run_mc_diagnostic(ps, t, DIAG[name][0], KT, 0.12)
+            for key, v in zip(DIAG_KEYS, rec):
+                out[name + "/diag_" + key] = np.array(v)
+            out[name + "/diag_steps"] = np.array(DIAG[name][0])
+            out[name + "/diag_seed"] = np.array(DIAG[name][1])
         print(name, len(ps), "atoms, total", out[name + "/potential"])
     np.savez_compressed(os.path.join(HERE, "pdb_golden.npz"), **out)
 

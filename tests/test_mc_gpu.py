@@ -139,3 +139,14 @@ def test_capacity_growth_keeps_the_trajectory(ctx, monkeypatch, caps):
     for a, b in zip(got["stats"], ref["stats"]):
         assert a["accepted"] == b["accepted"] and a["rebuilds"] == b["rebuilds"], (a, b)
         assert abs(a["cur_e"] - b["cur_e"]) <= 1e-9 * max(1.0, abs(b["cur_e"]))
+
+
+def test_mc_diagnostic_rejects_bad_arguments():
+    import protein_physics as PP
+    s = S.chain(24, 3)
+    t = S.topology(s)
+    eng = PP.PhysicsEngine()
+    with pytest.raises(ValueError):
+        eng.run_mc_diagnostic(S.particles(s), t, 0)
+    with pytest.raises(ValueError):
+        eng.run_mc_diagnostic([], t, 10)

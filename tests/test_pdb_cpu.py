@@ -163,3 +163,26 @@ def test_oracle_energy_matches_reference(golden, parsed):
         assert abs(got[term] - ref[k]) <= 1e-11 * max(abs(ref[k]), 1e-300) or got[term] == ref[k], \
             (name, term, got[term], ref[k])
     assert abs(got["TOTAL"] - float(golden[name + "/potential"])) <= 1e-11 * abs(float(golden[name + "/potential"]))
+
+
+DIAG_KEYS = ("move_type", "proposed_delta", "accepted", "d_hardcore", "d_nonbonded_soft", "d_sasa", "d_dih",
+             "d_ss")
+
+
+@pytest.mark.parametrize("name", ("1XQ8", "SM_P01308"))
+def test_oracle_mc_diagnostic_matches_reference(golden, name):
+    """run_mc_diagnostic records (seeded member generator) from the
+    reference module against the oracle: move types and accept decisions
+    exact, |delta| exact, energy changes within 1e-9 of the move's scale."""
+    import protein_physics as PP
+    s = pdb.load(_path(name))
+    c = O.topo_to_csr(pdb.topology(s, PP))
+    steps, seed = int(golden[name + "/diag_steps"]), int(golden[name + "/diag_seed"])
+    got = O.mc_diagnostic(s, c, steps, 0.0019872041 * 300.0, 0.12, seed=seed)
+    ref = {k: golden[name + "/diag_" + k] for k in DIAG_KEYS}
+    assert np.array_equal(got["move_type"], ref["move_type"])
+    assert np.array_equal(got["accepted"], ref["accepted"])
+    assert np.array_equal(got["proposed_delta"], ref["proposed_delta"])
+    scale = sum(np.abs(ref[k]) for k in DIAG_KEYS[3:]) + 1e-12
+    for k in DIAG_KEYS[3:]:
+        assert np.all(np.abs(got[k] - ref[k]) <= 1e-9 * scale), (name, k)

@@ -88,3 +88,41 @@ def test_mc_trajectory_matches_oracle(ctx, name, mode):
     assert np.max(np.abs(gpu["xyz"][0] - orc["xyz"])) < 1e-8, name
     o2 = O.mc_run(s, c, steps, KT, 0.12, seed=2025, mode=mode)
     assert np.max(np.abs(gpu["xyz"][1] - o2["xyz"])) < 1e-8, name
+
+
+DIAG_KEYS = ("move_type", "proposed_delta", "accepted", "d_hardcore", "d_nonbonded_soft", "d_sasa", "d_dih",
+             "d_ss")
+
+
+@pytest.mark.parametrize("name,steps,seed", [("SM_P01308", 200, 57), ("1XQ8", 120, 31)])
+def test_mc_diagnostic_matches_oracle(ctx, name, steps, seed):
+    """run_mc_diagnostic on the device against the oracle's (itself pinned to
+    the reference's records): same logged moves, |delta| and decisions,
+    energy changes within 1e-5 of each move's per-item scale."""
+    s, t, c = _parsed(name)
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    got = ctx.mc_diagnostic(steps, temperature=KT, max_angle=0.12, seed=seed)
+    orc = O.mc_diagnostic(s, c, steps, KT, 0.12, seed=seed, l1=True)
+    assert np.array_equal(got["move_type"], orc["move_type"]), name
+    assert np.array_equal(got["accepted"], orc["accepted"]), name
+    assert np.array_equal(got["proposed_delta"], orc["proposed_delta"]), name
+    tol = 1e-5 * orc["dE_l1"] + 1e-9
+    for k in DIAG_KEYS[3:]:
+        assert np.all(np.abs(got[k] - orc[k]) <= tol), (name, k)
+
+
+def test_dropin_run_mc_diagnostic(golden):
+    """The drop-in method against the reference module's own records
+    (seeded): the tuple of 8 lists."""
+    import protein_physics as PP
+    s, t, c = _parsed("SM_P01308")
+    rec = PP.PhysicsEngine().run_mc_diagnostic(pdb.particles(s, PP), t, 200, KT, 0.12, seed=57)
+    assert len(rec) == 8
+    ref = [golden["SM_P01308/diag_" + k] for k in DIAG_KEYS]
+    assert list(rec[0]) == ref[0].tolist() and list(rec[2]) == ref[2].tolist()
+    assert np.array_equal(np.array(rec[1]), ref[1])
+    # error scale of each move: sum of |per-item change| (from the oracle's
+    # replay of the same moves)
+    tol = 1e-5 * O.mc_diagnostic(s, c, 200, KT, 0.12, seed=57, l1=True)["dE_l1"] + 1e-9
+    for got, want in zip(rec[3:], ref[3:]):
+        assert np.all(np.abs(np.array(got) - want) <= tol)
