@@ -92,6 +92,20 @@ enum {
                                alma_mc_diagnostic */
 };
 
+/* Metropolis rule of every torsion-MC mode: accept when dE < 0, else draw
+ * u = uniform_real(0, 1) from the chain's generator and accept when
+ * u < exp(-dE / kT) (src/physics_engine.cpp:2387, 2489, 2656, 2734, 2866,
+ * 3051).  A move whose |dE| <= ALMA_NULL_DE kcal/mol is taken as dE = 0
+ * (drawn, accepted).  Such moves are null moves: a rotation whose moving
+ * side is only the bond's own far atom (a hydroxyl or thiol rotor of a
+ * heavy-atom model) moves that atom by rounding alone, and dE is rounding
+ * noise of the energy sums (|dE| up to ~1e-13).  The reference's branch
+ * there (draw or not) follows the sign of that noise, which depends on its
+ * summation order and build flags; this library's branch is the fixed one
+ * above.  The move is accepted either way; what can differ is whether a
+ * draw is consumed, and with it the generator stream of the later moves. */
+#define ALMA_NULL_DE 1e-10
+
 typedef struct {
     int32_t mode;
     int32_t n_chains;        /* independent chains; chain c draws from mt19937(seed + c) */
@@ -140,6 +154,35 @@ alma_status alma_mc_diagnostic(alma_ctx* ctx, int32_t n_steps, double temperatur
                                uint32_t seed, int32_t* n_records, int32_t* move_type, double* proposed_delta,
                                int32_t* accepted, double* d_hardcore, double* d_soft, double* d_sasa,
                                double* d_dih, double* d_ss);
+
+/* Replica exchange over landscape segments: the GUI's parallel-tempering
+ * driver (python/gui_main.py:1960-2013, _PTRunnable._run_pt) on the device.
+ * n_replicas slots at kT = temperature * ladder_ratio^k run together as MC
+ * chains; slot k draws from one mt19937(seed + k) for the whole run (the
+ * GUI's per-slot engine).  max(1, n_snapshots / swap_interval) segments:
+ * each slot runs run_landscape_segment semantics (a fresh list build,
+ * swap_interval snapshots of steps_per_snap steps, fresh calculate_potential
+ * per snapshot) from its configuration with its configuration's tuned step;
+ * then pairs (0,1),(2,3).. on even segments, (1,2),.. on odd, swap
+ * configuration, energy and step when numpy default_rng(exchange_seed)
+ * .random() < min(1, exp((E_a - E_b)(1/T_a - 1/T_b))).  Slot 0's snapshots
+ * and energies are pooled into pool_xyz (3n each) / pool_e, *n_pool of them
+ * (segments * swap_interval); swaps: attempted, accepted. */
+typedef struct {
+    int32_t n_replicas;
+    int32_t n_snapshots;
+    int32_t steps_per_snap;
+    int32_t swap_interval;
+    double temperature;
+    double ladder_ratio;
+    double max_angle;
+    uint32_t seed;
+    uint32_t pad;
+    uint64_t exchange_seed;
+} alma_landscape_pt_params;
+
+alma_status alma_landscape_pt(alma_ctx* ctx, const alma_landscape_pt_params* p, double* pool_xyz, double* pool_e,
+                              int32_t* n_pool, int64_t* swaps);
 
 /* Measurement of the last alma_mc_run (no reference counterpart): out[12] =
  * {device MC kernel ms (HIP events on the context stream), MC launches,

@@ -395,7 +395,10 @@ struct Chain {
     double last_cross[2] = {0, 0}, last_hc[2] = {0, 0};
 
     Chain(const Sys& sys, const double* x0, const orc_topo* topo, uint32_t seed, double max_angle)
-        : s(sys), t(topo), x(x0, x0 + 3 * (size_t)sys.n), a(sys.n), rng(seed), cur_max(max_angle),
+        : Chain(sys, x0, topo, MT(seed), max_angle) {}
+    // a chain continuing an existing generator (one engine across calls)
+    Chain(const Sys& sys, const double* x0, const orc_topo* topo, const MT& gen, double max_angle)
+        : s(sys), t(topo), x(x0, x0 + 3 * (size_t)sys.n), a(sys.n), rng(gen), cur_max(max_angle),
           in_side(sys.n, 0), group(sys.n, 0) {
         nl.build(s.n, x.data());
         born_full(s.n, x.data(), s.rs, nl, a.data(), nullptr);
@@ -486,7 +489,13 @@ struct Chain {
         return L;
     }
 
-    bool metropolis(double dE, double T) { return dE < 0.0 || rng.canonical() < std::exp(-dE / T); }
+    // Metropolis test (uni(gen) drawn only when dE >= 0); |dE| <= null_floor
+    // is taken as 0 (0 keeps the reference's rule exactly)
+    double null_floor = 0.0;
+    bool metropolis(double dE, double T) {
+        if (std::fabs(dE) <= null_floor) dE = 0.0;
+        return dE < 0.0 || rng.canonical() < std::exp(-dE / T);
+    }
 
     void restore(const std::vector<int>& idx, const std::vector<double>& op,
                  const std::vector<double>& ob) {
@@ -1114,6 +1123,7 @@ int orc_mc_run(int n, double* xyz, const double* q, const double* rstar, const d
     if (n <= 0 || !topo || topo->n_rot <= 0 || p->steps <= 0) return -1;
     Sys s{n, q, rstar, eps};
     Chain ch(s, xyz, topo, p->seed, p->max_angle);
+    ch.null_floor = p->null_floor;
     for (int64_t k = 0; k < p->skip; ++k) ch.rng();
     ch.rng.used = 0;
     ch.want_items = tr && tr->dE_l1;
@@ -1213,6 +1223,136 @@ int orc_mc_run(int n, double* xyz, const double* q, const double* rstar, const d
     return 0;
 }
 
+// numpy.random.default_rng(seed) for an integer seed (numpy/random/
+// bit_generator.pyx SeedSequence, numpy/random/src/pcg64: PCG64 XSL-RR 128/64)
+// and Generator.random() = (next64 >> 11) * 2^-53.
+struct Pcg64 {
+    unsigned __int128 state = 0, inc = 0;
+    static constexpr uint32_t kInitA = 0x43b0d7e5u, kMultA = 0x931e8875u, kInitB = 0x8b51f9ddu,
+                              kMultB = 0x58f38dedu, kMixL = 0xca01f9ddu, kMixR = 0x4973f715u;
+    explicit Pcg64(uint64_t seed) {
+        std::vector<uint32_t> ent;   // the seed's 32-bit words, least significant first
+        uint64_t v = seed;
+        do {
+            ent.push_back((uint32_t)v);
+            v >>= 32;
+        } while (v);
+        uint32_t hc = kInitA;
+        auto hashmix = [&](uint32_t x) {
+            x ^= hc;
+            hc *= kMultA;
+            x *= hc;
+            x ^= x >> 16;
+            return x;
+        };
+        auto mix = [](uint32_t x, uint32_t y) {
+            uint32_t r = kMixL * x - kMixR * y;
+            return r ^ (r >> 16);
+        };
+        uint32_t pool[4];
+        for (int i = 0; i < 4; ++i) pool[i] = hashmix(i < (int)ent.size() ? ent[i] : 0u);
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+                if (a != b) pool[b] = mix(pool[b], hashmix(pool[a]));
+        for (size_t a = 4; a < ent.size(); ++a)
+            for (int b = 0; b < 4; ++b) pool[b] = mix(pool[b], hashmix(ent[a]));
+        uint32_t w[8], hb = kInitB;   // generate_state(4, uint64)
+        for (int i = 0; i < 8; ++i) {
+            uint32_t x = pool[i % 4] ^ hb;
+            hb *= kMultB;
+            x *= hb;
+            w[i] = x ^ (x >> 16);
+        }
+        uint64_t u[4];
+        for (int i = 0; i < 4; ++i) u[i] = (uint64_t)w[2 * i] | ((uint64_t)w[2 * i + 1] << 32);
+        const unsigned __int128 init_state = ((unsigned __int128)u[0] << 64) | u[1];
+        const unsigned __int128 init_seq = ((unsigned __int128)u[2] << 64) | u[3];
+        inc = (init_seq << 1) | 1u;
+        step();
+        state += init_state;
+        step();
+    }
+    void step() {
+        const unsigned __int128 mult = ((unsigned __int128)0x2360ed051fc65da4ull << 64) | 0x4385df649fccf645ull;
+        state = state * mult + inc;
+    }
+    uint64_t next64() {
+        step();
+        const uint64_t x = (uint64_t)(state >> 64) ^ (uint64_t)state;
+        const unsigned rot = (unsigned)(state >> 122);
+        return (x >> rot) | (x << ((64u - rot) & 63u));
+    }
+    double random() { return (double)(next64() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+void orc_pcg64_random(uint64_t seed, int n, double* out) {
+    Pcg64 g(seed);
+    for (int k = 0; k < n; ++k) out[k] = g.random();
+}
+
+// _PTRunnable._run_pt, python/gui_main.py:1960-2013, with each slot's
+// run_landscape_segment (src/physics_engine.cpp:2952-3270) as a chain that
+// starts from the slot's configuration and step and continues the slot's
+// engine generator.
+int orc_landscape_pt(int n, const double* xyz0, const double* q, const double* rstar, const double* eps,
+                     const orc_topo* topo, const orc_lpt_params* p, double* pool_xyz, double* pool_e,
+                     int64_t* swaps) {
+    if (n <= 0 || !topo || topo->n_rot <= 0 || !p || p->n_replicas <= 0 || p->n_snapshots <= 0 ||
+        p->steps_per_snap <= 0 || p->swap_interval <= 0)
+        return -1;
+    Sys s{n, q, rstar, eps};
+    const int R = p->n_replicas;
+    std::vector<double> ladder(R);
+    for (int k = 0; k < R; ++k) ladder[k] = p->temperature * std::pow(p->ladder_ratio, (double)k);
+    std::vector<MT> gens;
+    for (int k = 0; k < R; ++k) gens.emplace_back(p->seed + (uint32_t)k);
+    std::vector<std::vector<double>> cfg(R, std::vector<double>(xyz0, xyz0 + 3 * (size_t)n));
+    std::vector<double> E(R, 0.0), cm(R, p->max_angle);
+    Pcg64 ex(p->exchange_seed);
+    const int nseg = std::max(1, p->n_snapshots / p->swap_interval);
+    int64_t att = 0, acc = 0;
+    size_t pooled = 0;
+    for (int seg = 0; seg < nseg; ++seg) {
+        for (int k = 0; k < R; ++k) {
+            Chain ch(s, cfg[k].data(), topo, gens[k], cm[k]);
+            ch.null_floor = p->null_floor;
+            double e_last = 0.0;
+            for (int sn = 0; sn < p->swap_interval; ++sn) {
+                for (int st = 0; st < p->steps_per_snap; ++st) ch.step(ORC_MODE_LANDSCAPE, ladder[k]);
+                double terms[ORC_NTERMS];
+                orc_energy_terms(n, ch.x.data(), q, rstar, eps, topo, terms);
+                e_last = terms[ORC_TOTAL];
+                if (k == 0) {
+                    if (pool_xyz) std::copy(ch.x.begin(), ch.x.end(), pool_xyz + pooled * 3 * n);
+                    if (pool_e) pool_e[pooled] = e_last;
+                    ++pooled;
+                }
+            }
+            cfg[k] = ch.x;
+            E[k] = e_last;
+            cm[k] = ch.cur_max;
+            gens[k] = ch.rng;
+        }
+        for (int a = seg % 2; a + 1 < R; a += 2) {
+            const int b = a + 1;
+            const double x = (E[a] - E[b]) * ((1.0 / ladder[a]) - (1.0 / ladder[b]));
+            const double p_swap = x >= 0 ? 1.0 : std::exp(x);
+            ++att;
+            if (ex.random() < p_swap) {
+                std::swap(cfg[a], cfg[b]);
+                std::swap(E[a], E[b]);
+                std::swap(cm[a], cm[b]);
+                ++acc;
+            }
+        }
+    }
+    if (swaps) {
+        swaps[0] = att;
+        swaps[1] = acc;
+    }
+    return 0;
+}
+
 int orc_mc_multi(int nchains, int n, const double* xyz0, const double* q, const double* rstar,
                  const double* eps, const orc_topo* topo, const orc_mc_params* p, double* xyz_out,
                  orc_mc_stats* st_out) {
@@ -1221,6 +1361,7 @@ int orc_mc_multi(int nchains, int n, const double* xyz0, const double* q, const 
 #pragma omp parallel for schedule(dynamic)
     for (int c = 0; c < nchains; ++c) {
         Chain ch(s, xyz0, topo, p->seed + (uint32_t)c, p->max_angle);
+        ch.null_floor = p->null_floor;
         for (int k = 0; k < p->steps; ++k) ch.step(p->mode, p->temperature);
         if (xyz_out) std::copy(ch.x.begin(), ch.x.end(), xyz_out + (size_t)c * 3 * n);
         if (st_out) {

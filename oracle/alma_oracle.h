@@ -72,6 +72,12 @@ typedef struct {
     uint32_t seed;          /* mt19937 seed                                    */
     int64_t skip;           /* words discarded before the chain starts (a later
                                candidate of the serial generate_ensemble loop) */
+    double null_floor;      /* Metropolis: |dE| <= null_floor counts as dE = 0
+                               (the draw is taken).  0 = the reference's rule,
+                               draw iff dE >= 0, whose branch on a null move
+                               (a rotation that moves no atom off its axis:
+                               dE is rounding noise, +-1e-13) follows the sign
+                               of that noise. */
 } orc_mc_params;
 
 typedef struct {
@@ -170,6 +176,39 @@ int orc_pt_run(int n_prot, const int* n_atoms, const double* q, const double* rs
                int* tr_bond, double* tr_delta, double* tr_dE, int* tr_acc, int* tr_slot);
 
 int orc_num_threads(void);
+
+/* ── parallel tempering over landscape segments ────────────────────────────
+ * The GUI's PT driver (python/gui_main.py:1960-2013): n_replicas slots at
+ * kT = temperature * ladder_ratio^k; per segment every slot runs
+ * run_landscape_segment (swap_interval snapshots x steps_per_snap steps) from
+ * its configuration with its configuration's tuned step, slot k's engine
+ * drawing from one mt19937(seed + k) across segments; then alternating-pair
+ * exchanges ((0,1),(2,3).. on even segments, (1,2),.. on odd) accepted when
+ * numpy's default_rng(exchange_seed).random() < min(1, exp((E_a - E_b)
+ * (1/T_a - 1/T_b))), swapping configuration, energy and tuned step.  Slot 0's
+ * snapshots and fresh energies are pooled: max(1, n_snapshots /
+ * swap_interval) * swap_interval of them. */
+typedef struct {
+    int n_replicas;
+    int n_snapshots;
+    int steps_per_snap;
+    int swap_interval;
+    double temperature;
+    double ladder_ratio;
+    double max_angle;
+    uint32_t seed;
+    uint32_t pad;
+    uint64_t exchange_seed;
+    double null_floor;      /* as in orc_mc_params */
+} orc_lpt_params;
+
+int orc_landscape_pt(int n, const double* xyz0, const double* q, const double* rstar, const double* eps,
+                     const orc_topo* topo, const orc_lpt_params* p, double* pool_xyz, double* pool_e,
+                     int64_t* swaps);
+
+/* numpy.random.default_rng(seed).random(), n draws (SeedSequence -> PCG64
+ * XSL-RR, (next64 >> 11) * 2^-53) */
+void orc_pcg64_random(uint64_t seed, int n, double* out);
 
 #ifdef __cplusplus
 }

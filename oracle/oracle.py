@@ -34,7 +34,7 @@ class Topo(C.Structure):
 class McParams(C.Structure):
     _fields_ = [("mode", C.c_int), ("steps", C.c_int), ("snapshot_every", C.c_int),
                 ("temperature", C.c_double), ("max_angle", C.c_double),
-                ("seed", C.c_uint32), ("skip", C.c_int64)]
+                ("seed", C.c_uint32), ("skip", C.c_int64), ("null_floor", C.c_double)]
 
 
 class McStats(C.Structure):
@@ -188,11 +188,11 @@ def dihedral_angle(q):
 
 
 def mc_run(sysd, topo_csr, steps, temperature=0.6, max_angle=0.12, seed=5489,
-           mode=MODE_ENSEMBLE, snapshot_every=0, trace=False, l1=False, skip=0):
+           mode=MODE_ENSEMBLE, snapshot_every=0, trace=False, l1=False, skip=0, null_floor=0.0):
     xyz = _f64(sysd["xyz"]).copy()
     n = len(xyz)
     th = TopoHandle(topo_csr, n)
-    p = McParams(mode, steps, snapshot_every, temperature, max_angle, seed, skip)
+    p = McParams(mode, steps, snapshot_every, temperature, max_angle, seed, skip, null_floor)
     st = McStats()
     tr = None
     arrays = {}
@@ -226,7 +226,8 @@ def mc_run(sysd, topo_csr, steps, temperature=0.6, max_angle=0.12, seed=5489,
     return res
 
 
-def mc_diagnostic(sysd, topo_csr, n_steps, temperature=0.6, max_angle=0.12, seed=5489, l1=False):
+def mc_diagnostic(sysd, topo_csr, n_steps, temperature=0.6, max_angle=0.12, seed=5489, l1=False,
+                  null_floor=0.0):
     """run_mc_diagnostic (src/physics_engine.cpp:3300-3667) seeded with
     mt19937(seed): one record per logged torsion / crankshaft move --
     move_type, |delta|, accepted, d_hardcore, d_nonbonded_soft, d_sasa,
@@ -235,7 +236,7 @@ def mc_diagnostic(sysd, topo_csr, n_steps, temperature=0.6, max_angle=0.12, seed
     xyz = _f64(sysd["xyz"]).copy()
     n = len(xyz)
     th = TopoHandle(topo_csr, n)
-    p = McParams(MODE_DIAGNOSTIC, n_steps, 0, temperature, max_angle, seed, 0)
+    p = McParams(MODE_DIAGNOSTIC, n_steps, 0, temperature, max_angle, seed, 0, null_floor)
     st = McStats()
     a = {"kind": np.zeros(n_steps, np.int32), "bond": np.zeros(n_steps, np.int32),
          "delta": np.zeros(n_steps), "dE": np.zeros(n_steps), "dE_l1": np.zeros(n_steps) if l1 else None,
@@ -262,11 +263,11 @@ def mc_diagnostic(sysd, topo_csr, n_steps, temperature=0.6, max_angle=0.12, seed
 
 
 def mc_multi(sysd, topo_csr, nchains, steps, temperature=0.6, max_angle=0.12, seed=5489,
-             mode=MODE_ENSEMBLE):
+             mode=MODE_ENSEMBLE, null_floor=0.0):
     xyz0 = _f64(sysd["xyz"])
     n = len(xyz0)
     th = TopoHandle(topo_csr, n)
-    p = McParams(mode, steps, 0, temperature, max_angle, seed)
+    p = McParams(mode, steps, 0, temperature, max_angle, seed, 0, null_floor)
     out = np.zeros((nchains, n, 3))
     st = (McStats * nchains)()
     rc = lib().orc_mc_multi(nchains, n, _p(xyz0, C.c_double), _p(_f64(sysd["charge"]), C.c_double),
@@ -402,3 +403,41 @@ def topo_to_csr(t) -> dict:
         "cp_phi": np.array([p[0] for p in cp], np.int32), "cp_psi": np.array([p[1] for p in cp], np.int32),
         "csp_a": np.array([p[0] for p in csp], np.int32), "csp_b": np.array([p[1] for p in csp], np.int32),
     }
+
+
+class LptParams(C.Structure):
+    _fields_ = [("n_replicas", C.c_int), ("n_snapshots", C.c_int), ("steps_per_snap", C.c_int),
+                ("swap_interval", C.c_int), ("temperature", C.c_double), ("ladder_ratio", C.c_double),
+                ("max_angle", C.c_double), ("seed", C.c_uint32), ("pad", C.c_uint32),
+                ("exchange_seed", C.c_uint64), ("null_floor", C.c_double)]
+
+
+def landscape_pt(sysd, topo_csr, n_replicas, n_snapshots, steps_per_snap, temperature, ladder_ratio,
+                 swap_interval, max_angle=0.12, seed=5489, exchange_seed=0, null_floor=0.0):
+    """The GUI's parallel-tempering driver over landscape segments
+    (python/gui_main.py:1960-2013): slot 0's pooled snapshots and fresh
+    energies, and the exchange counts (attempted, accepted)."""
+    xyz0 = _f64(sysd["xyz"])
+    n = len(xyz0)
+    th = TopoHandle(topo_csr, n)
+    nseg = max(1, n_snapshots // swap_interval)
+    m = nseg * swap_interval
+    px = np.zeros((m, n, 3))
+    pe = np.zeros(m)
+    sw = np.zeros(2, np.int64)
+    p = LptParams(n_replicas, n_snapshots, steps_per_snap, swap_interval, temperature, ladder_ratio, max_angle,
+                  seed, 0, exchange_seed, null_floor)
+    rc = lib().orc_landscape_pt(n, _p(xyz0, C.c_double), _p(_f64(sysd["charge"]), C.c_double),
+                                _p(_f64(sysd["rstar"]), C.c_double), _p(_f64(sysd["eps"]), C.c_double), th.ptr(),
+                                C.byref(p), _p(px, C.c_double), _p(pe, C.c_double),
+                                sw.ctypes.data_as(C.POINTER(C.c_int64)))
+    if rc != 0:
+        raise ValueError("orc_landscape_pt rejected its arguments")
+    return {"snapshots": px, "energies": pe, "swaps": sw}
+
+
+def pcg64_random(seed, n):
+    """numpy.random.default_rng(seed).random(n), restated."""
+    out = np.zeros(n)
+    lib().orc_pcg64_random(C.c_uint64(seed), n, _p(out, C.c_double))
+    return out

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_PKG, "libalma.so")
 ALMA_OK, ALMA_EINVAL, ALMA_ERUNTIME, ALMA_ENODEV = 0, 1, 2, 3
 TERMS = ("DH", "GB", "LJ", "HC", "SASA", "DIH", "SS", "TOTAL")
 MC_ENSEMBLE, MC_LANDSCAPE, MC_TORSION30, MC_MALA = 0, 1, 2, 3
+NULL_DE = 1e-10   # ALMA_NULL_DE: |dE| at or below it is a null move (taken as 0)
 
 _I = C.POINTER(C.c_int32)
 _D = C.POINTER(C.c_double)
@@ -44,6 +45,13 @@ class McParams(C.Structure):
 class McStats(C.Structure):
     _fields_ = [("cur_e", C.c_double), ("cur_max", C.c_double), ("accepted", C.c_int64),
                 ("moves", C.c_int64), ("rebuilds", C.c_int64), ("words", C.c_int64)]
+
+
+class LptParams(C.Structure):
+    _fields_ = [("n_replicas", C.c_int32), ("n_snapshots", C.c_int32), ("steps_per_snap", C.c_int32),
+                ("swap_interval", C.c_int32), ("temperature", C.c_double), ("ladder_ratio", C.c_double),
+                ("max_angle", C.c_double), ("seed", C.c_uint32), ("pad", C.c_uint32),
+                ("exchange_seed", C.c_uint64)]
 
 
 class PtSystems(C.Structure):
@@ -96,6 +104,7 @@ def lib():
         L.alma_mc_trace.argtypes = [P, _I, _I, _D, _D, _I, _D]
         L.alma_forces.argtypes = [P, _D, _D, _D]
         L.alma_mc_timing.argtypes = [P, _D]
+        L.alma_landscape_pt.argtypes = [P, C.POINTER(LptParams), _D, _D, _I, C.POINTER(C.c_int64)]
         L.alma_mc_diagnostic.argtypes = [P, C.c_int32, C.c_double, C.c_double, C.c_uint32, _I, _I, _D, _I, _D,
                                          _D, _D, _D, _D]
         L.alma_pt_set.argtypes = [P, C.POINTER(PtSystems), _D]
@@ -109,7 +118,8 @@ def lib():
 
 EXPORTED = ("alma_create", "alma_destroy", "alma_last_error", "alma_device_name",
             "alma_set_system", "alma_set_positions", "alma_energy", "alma_nlist",
-            "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_mc_timing", "alma_mc_diagnostic", "alma_forces",
+            "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_mc_timing", "alma_mc_diagnostic",
+            "alma_landscape_pt", "alma_forces",
             "alma_pt_set", "alma_pt_run", "alma_pt_trace", "alma_pt_energy", "alma_pt_timing")
 
 
@@ -264,6 +274,23 @@ class Context:
             _p(a["d_hardcore"], C.c_double), _p(a["d_nonbonded_soft"], C.c_double), _p(a["d_sasa"], C.c_double),
             _p(a["d_dih"], C.c_double), _p(a["d_ss"], C.c_double)))
         return {k: v[:int(mc[0])] for k, v in a.items()}
+
+    def landscape_pt(self, n_replicas, n_snapshots, steps_per_snap, temperature=0.6, ladder_ratio=1.6,
+                     swap_interval=5, max_angle=0.12, seed=5489, exchange_seed=0) -> dict:
+        """The GUI's replica-exchange driver over landscape segments, all
+        slots on the device: slot 0's pooled snapshots and fresh energies and
+        the exchange counts (attempted, accepted)."""
+        nseg = max(1, n_snapshots // swap_interval)
+        m = nseg * swap_interval
+        px = np.zeros((m, self.n, 3))
+        pe = np.zeros(m)
+        cnt = np.zeros(1, np.int32)
+        sw = np.zeros(2, np.int64)
+        p = LptParams(n_replicas, n_snapshots, steps_per_snap, swap_interval, temperature, ladder_ratio, max_angle,
+                      seed, 0, exchange_seed)
+        self._check(lib().alma_landscape_pt(self._h, C.byref(p), _p(px, C.c_double), _p(pe, C.c_double),
+                                            _p(cnt, C.c_int32), sw.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {"snapshots": px[:int(cnt[0])], "energies": pe[:int(cnt[0])], "swaps": sw}
 
     def mc_timing(self) -> dict:
         out = np.zeros(12)

@@ -504,8 +504,82 @@ void grow_for_overflow(ChainBatch& b, std::vector<ChainState>& st) {
 
 
 
+// numpy.random.default_rng(seed).random() for an integer seed: SeedSequence
+// pool mixing and state generation (numpy/random/bit_generator.pyx), then
+// PCG64 XSL-RR 128/64 (numpy/random/src/pcg64) and (next64 >> 11) * 2^-53.
+// The GUI draws its replica-exchange decisions from this generator.
+class NumpyPcg64 {
+public:
+    explicit NumpyPcg64(uint64_t seed) {
+        uint32_t words[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+        const int nw = (seed >> 32) ? 2 : 1;
+        uint32_t hash = 0x43b0d7e5u;
+        auto hashmix = [&hash](uint32_t v) {
+            v ^= hash;
+            hash *= 0x931e8875u;
+            v *= hash;
+            return v ^ (v >> 16);
+        };
+        auto mix = [](uint32_t x, uint32_t y) {
+            const uint32_t r = 0xca01f9ddu * x - 0x4973f715u * y;
+            return r ^ (r >> 16);
+        };
+        uint32_t pool[4];
+        for (int i = 0; i < 4; ++i) pool[i] = hashmix(i < nw ? words[i] : 0u);
+        for (int src = 0; src < 4; ++src)
+            for (int dst = 0; dst < 4; ++dst)
+                if (src != dst) pool[dst] = mix(pool[dst], hashmix(pool[src]));
+        uint64_t st[4];
+        uint32_t h = 0x8b51f9ddu;
+        for (int i = 0; i < 8; ++i) {
+            uint32_t v = pool[i & 3] ^ h;
+            h *= 0x58f38dedu;
+            v *= h;
+            v ^= v >> 16;
+            if (i & 1)
+                st[i >> 1] |= (uint64_t)v << 32;
+            else
+                st[i >> 1] = v;
+        }
+        inc_ = (((unsigned __int128)st[2] << 64 | st[3]) << 1) | 1u;
+        state_ = 0;
+        advance();
+        state_ += (unsigned __int128)st[0] << 64 | st[1];
+        advance();
+    }
+    double random() {
+        advance();
+        const uint64_t x = (uint64_t)(state_ >> 64) ^ (uint64_t)state_;
+        const unsigned r = (unsigned)(state_ >> 122);
+        const uint64_t out = (x >> r) | (x << ((64u - r) & 63u));
+        return (double)(out >> 11) * 0x1.0p-53;
+    }
+
+private:
+    void advance() {
+        static const unsigned __int128 kMul = (unsigned __int128)0x2360ed051fc65da4ull << 64 | 0x4385df649fccf645ull;
+        state_ = state_ * kMul + inc_;
+    }
+    unsigned __int128 state_ = 0, inc_ = 0;
+};
+
+// Parallel tempering over landscape segments (python/gui_main.py:1960-2013)
+// run by run_mc: chain k is temperature slot k (its mt19937 stream continues
+// across segments, as the GUI's per-slot engine does); every segment starts
+// with a list build from the slot's configuration and its configuration's
+// tuned step, takes `per_seg` snapshots, then alternating-pair exchanges swap
+// configuration, energy and tuned step.
+struct LandscapePt {
+    int n_seg = 0;
+    std::vector<double> ladder;
+    uint64_t exchange_seed = 0;
+    double* pool_xyz = nullptr;   // slot 0: n_seg * snapshots per segment, 3n each
+    double* pool_e = nullptr;
+    int64_t* swaps = nullptr;     // attempted, accepted
+};
+
 void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_stats* stats, double* snap_xyz,
-            double* snap_e) {
+            double* snap_e, const LandscapePt* pt = nullptr) {
     DevSys& s = ctx->sys;
     if (s.n == 0) throw Fail{ALMA_EINVAL, "initial_state empty"};
     if (!s.has_topo || s.topo.n_rot <= 0) throw Fail{ALMA_EINVAL, "topology has no rotatable bonds"};
@@ -589,6 +663,18 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     // (status CS_REBUILD at launch; not counted as a rebuild)
 
     std::vector<double> snap_pos(4 * (size_t)n);
+    std::vector<double> seg_xyz, seg_e;
+    if (pt) {
+        // snapshots of every slot in a segment, for the exchanges and the pool
+        seg_xyz.resize((size_t)nch * nsnap * 3 * n);
+        seg_e.resize((size_t)nch * nsnap);
+        snap_xyz = seg_xyz.data();
+        snap_e = seg_e.data();
+    }
+    std::unique_ptr<NumpyPcg64> exch(pt ? new NumpyPcg64(pt->exchange_seed) : nullptr);
+    std::vector<double> cm(nch, p.max_angle);
+    int64_t att = 0, acc = 0;
+    for (int seg = 0; seg < (pt ? pt->n_seg : 1); ++seg) {
     for (int round = 0;; ++round) {
         HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
         std::vector<int> run;
@@ -659,6 +745,60 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
             ctx->t_mc_ms += ms;
             ctx->n_mc_launch += 1;
         }
+    }
+    if (!pt) break;
+    // end of a segment: pool slot 0, then the GUI's exchange round
+    {
+        HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
+        const size_t frame = 3 * (size_t)n;
+        if (pt->pool_xyz)
+            std::copy(seg_xyz.begin(), seg_xyz.begin() + nsnap * frame, pt->pool_xyz + (size_t)seg * nsnap * frame);
+        if (pt->pool_e) std::copy(seg_e.begin(), seg_e.begin() + nsnap, pt->pool_e + (size_t)seg * nsnap);
+        std::vector<std::vector<double>> cfg(nch);
+        std::vector<double> E(nch);
+        for (int c = 0; c < nch; ++c) {
+            const double* last = seg_xyz.data() + ((size_t)c * nsnap + nsnap - 1) * frame;
+            cfg[c].assign(last, last + frame);
+            E[c] = seg_e[(size_t)c * nsnap + nsnap - 1];
+            cm[c] = st[c].cur_max;
+        }
+        for (int a = seg % 2; a + 1 < nch; a += 2) {
+            const int bb = a + 1;
+            const double x = (E[a] - E[bb]) * ((1.0 / pt->ladder[a]) - (1.0 / pt->ladder[bb]));
+            const double p_swap = x >= 0 ? 1.0 : std::exp(x);
+            ++att;
+            if (exch->random() < p_swap) {
+                std::swap(cfg[a], cfg[bb]);
+                std::swap(cm[a], cm[bb]);
+                ++acc;
+            }
+        }
+        if (seg + 1 == pt->n_seg) break;
+        // next segment: each slot restarts from its configuration with a
+        // fresh list, its configuration's step and a new tuning window; the
+        // slot's generator continues
+        std::vector<double> p4(4 * (size_t)n, 0.0);
+        for (int c = 0; c < nch; ++c) {
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) p4[4 * i + d] = cfg[c][3 * i + d];
+            HIP_TRY(hipMemcpy(b.pos + (size_t)c * 4 * n, p4.data(), sizeof(double) * p4.size(),
+                              hipMemcpyHostToDevice));
+            ChainState& x = st[c];
+            x.status = CS_REBUILD;
+            x.steps_done = 0;
+            x.cur_max = cm[c];
+            x.acc_win = x.tot_win = 0;
+            x.de_since = 0.0;
+            x.rb_pending = 0;
+            snaps_done[c] = 0;
+            x.steps_target = target_of(c);
+        }
+        HIP_TRY(hipMemcpy(b.st, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice));
+    }
+    }
+    if (pt && pt->swaps) {
+        pt->swaps[0] = att;
+        pt->swaps[1] = acc;
     }
     HIP_TRY(launch_chain_energy(b, s, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1616,6 +1756,40 @@ alma_status alma_mc_diagnostic(alma_ctx* ctx, int32_t n_steps, double temperatur
             if (d_dih) d_dih[k] = ctx->tr_parts[4 * k + 2];
             if (d_ss) d_ss[k] = ctx->tr_parts[4 * k + 3];
         }
+    });
+}
+
+alma_status alma_landscape_pt(alma_ctx* ctx, const alma_landscape_pt_params* p, double* pool_xyz, double* pool_e,
+                              int32_t* n_pool, int64_t* swaps) {
+    return guarded(ctx, [&] {
+        if (!p) throw Fail{ALMA_EINVAL, "parameters missing"};
+        if (p->n_replicas <= 0) throw Fail{ALMA_EINVAL, "n_replicas must be positive"};
+        if (p->n_snapshots <= 0 || p->steps_per_snap <= 0 || p->swap_interval <= 0)
+            throw Fail{ALMA_EINVAL, "n_snapshots/steps_per_snapshot/swap_interval must be positive"};
+        if (!(p->temperature > 0.0) || !(p->ladder_ratio > 0.0))
+            throw Fail{ALMA_EINVAL, "temperature and ladder ratio must be positive"};
+        HIP_TRY(hipSetDevice(ctx->device));
+        LandscapePt pt;
+        pt.n_seg = std::max(1, p->n_snapshots / p->swap_interval);
+        for (int k = 0; k < p->n_replicas; ++k) pt.ladder.push_back(p->temperature * std::pow(p->ladder_ratio, (double)k));
+        pt.exchange_seed = p->exchange_seed;
+        pt.pool_xyz = pool_xyz;
+        pt.pool_e = pool_e;
+        pt.swaps = swaps;
+        alma_mc_params mp{};
+        mp.mode = ALMA_MC_LANDSCAPE;
+        mp.n_chains = p->n_replicas;
+        mp.steps = p->swap_interval * p->steps_per_snap;
+        mp.snapshot_every = p->steps_per_snap;
+        mp.temperature = pt.ladder[0];
+        mp.max_angle = p->max_angle;
+        mp.mala_step = 0.02;
+        mp.seed = p->seed;
+        mp.chain_temperature = pt.ladder.data();
+        std::vector<double> xyz(3 * (size_t)ctx->sys.n * p->n_replicas);
+        std::vector<alma_mc_stats> st(p->n_replicas);
+        run_mc(ctx, mp, xyz.data(), st.data(), nullptr, nullptr, &pt);
+        if (n_pool) *n_pool = pt.n_seg * p->swap_interval;
     });
 }
 

@@ -1311,7 +1311,9 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 double dE = part[0] + part[1] + part[2] + part[3];
                 for (int q = 0; q < 4; ++q) mv.part[q] = part[q];
                 mv.dE = dE;
-                bool acc_ok = dE < 0.0 || W.canonical() < exp(-dE / cs->temperature);
+                // null moves (|dE| at the rounding floor) take the draw
+                const double dEm = fabs(dE) <= ALMA_NULL_DE ? 0.0 : dE;
+                bool acc_ok = dEm < 0.0 || W.canonical() < exp(-dEm / cs->temperature);
                 if (W.bad) {
                     acc_ok = false;
                     cs->status = CS_ERROR;

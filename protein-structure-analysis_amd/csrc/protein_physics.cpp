@@ -8,7 +8,8 @@
 //
 // Added beyond the reference (all optional): a `seed` keyword on the MC
 // methods (the reference seeds from std::random_device), energy_terms(),
-// born_radii(), neighbor_list(), forces() and mc_trace().
+// born_radii(), neighbor_list(), forces() and run_parallel_tempering() (the
+// GUI's replica-exchange loop on the device).
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
@@ -497,6 +498,46 @@ PYBIND11_MODULE(ALMA_PY_MODULE, m) {
             },
             py::arg("initial_state"), py::arg("topology"), py::arg("n_steps"), py::arg("temperature") = 0.6,
             py::arg("max_angle") = 0.12, py::arg("seed") = py::none())
+        .def(
+            "run_parallel_tempering",
+            // the GUI's replica-exchange driver (python/gui_main.py:1960-2013,
+            // defaults of :2296-2298) on the device: slot 0's pooled snapshots
+            // and fresh energies
+            [](Engine& self, const Parts& init, py::object topo, int n_snapshots, int steps_per_snap, double T,
+               double max_angle, int n_replicas, double ladder_ratio, int swap_interval, py::object rng_seed,
+               py::object seed) {
+                Csr c = csr_from_object(topo);
+                check_mc_args(init, c, n_snapshots, steps_per_snap, "n_snapshots/steps_per_snapshot must be positive");
+                alma_landscape_pt_params p{};
+                p.n_replicas = n_replicas;
+                p.n_snapshots = n_snapshots;
+                p.steps_per_snap = steps_per_snap;
+                p.swap_interval = swap_interval;
+                p.temperature = T;
+                p.ladder_ratio = ladder_ratio;
+                p.max_angle = max_angle;
+                p.seed = draw_seed(seed);
+                p.exchange_seed = rng_seed.is_none() ? ((uint64_t)std::random_device{}() << 32 | std::random_device{}())
+                                                     : rng_seed.cast<uint64_t>();
+                const int m = std::max(1, swap_interval > 0 ? n_snapshots / swap_interval : 1) *
+                              std::max(1, swap_interval);
+                std::vector<double> px((size_t)m * 3 * init.size()), pe(m);
+                int32_t got = 0;
+                int64_t sw[2] = {0, 0};
+                std::vector<Parts> snaps;
+                {
+                    py::gil_scoped_release rel;
+                    self.load(init, &c);
+                    self.check(alma_landscape_pt(self.ctx(), &p, px.data(), pe.data(), &got, sw));
+                    for (int k = 0; k < got; ++k) snaps.push_back(self.with_coords(init, px.data() + (size_t)k * 3 * init.size()));
+                }
+                pe.resize(got);
+                return std::make_pair(snaps, pe);
+            },
+            py::arg("initial_state"), py::arg("topology"), py::arg("n_snapshots"), py::arg("steps_per_snapshot"),
+            py::arg("temperature") = 0.6, py::arg("max_angle") = 0.12, py::arg("n_replicas") = 4,
+            py::arg("ladder_ratio") = 1.6, py::arg("swap_interval") = 5, py::arg("rng_seed") = py::none(),
+            py::arg("seed") = py::none())
         .def(
             "lowest_energy_structure",
             [](Engine& self, const std::vector<Parts>& ens) {

@@ -12,8 +12,10 @@ Sources, all in this container where the reference is present:
   * the reference module built from its own sources (oracle/_ref/alma_ref,
     see make_golden.py): BondTopology (build, concerted side-chain pairs,
     disulfides in _parse_pdb's order), the per-term energies / total of
-    calculate_potential for those particles, and seeded run_mc_diagnostic
-    records on two of them.
+    calculate_potential for those particles, seeded run_mc_diagnostic
+    records on two of them, and the GUI's replica-exchange loop
+    (gui_main.py:1960-2013, restated here over the module's
+    run_landscape_segment with seeded engines) on one.
 
 The structures are the reference's own data files (data/1XQ8.pdb: NMR with
 explicit hydrogens; data/SM_P01308.pdb: three disulfides; data/SM_P21543.pdb:
@@ -39,6 +41,9 @@ from alma import pdb  # noqa: E402
 STRUCTURES = ["1XQ8", "SM_P01308", "SM_P21543"]
 KT = 0.0019872041 * 300.0
 DIAG = {"1XQ8": (120, 31), "SM_P01308": (200, 57)}   # run_mc_diagnostic: steps, seed
+# replica exchange over landscape segments: replicas, snapshots, steps per
+# snapshot, swap interval, ladder ratio, max angle, engine seed, exchange seed
+PT = {"SM_P01308": (4, 6, 15, 3, 1.6, 0.12, 21, 4)}
 DIAG_KEYS = ("move_type", "proposed_delta", "accepted", "d_hardcore", "d_nonbonded_soft", "d_sasa", "d_dih",
              "d_ss")
 PROBES = [("XYZ", n) for n in ("CL1", "BR", "1HB", "OXT", "P1", "FE", "ZN1", "SE", "CA", "C7", "N3", "O1P",
@@ -46,6 +51,37 @@ PROBES = [("XYZ", n) for n in ("CL1", "BR", "1HB", "OXT", "P1", "FE", "ZN1", "SE
 PROBES += [(r, "CA") for r in ("MG", "CA", "ZN", "FE", "FE2", "FE3", "MN", "NI", "CU", "CO", "NA", "CL", "K", "MO")]
 PROBES += [("GLY", "HA"), ("PRO", "H"), ("PRO", "OXT"), ("ALA", "OXT"), ("HIS", "HE2"), ("MET", "H1"),
            ("ALA", "H2"), ("LYS", "H3"), ("CYS", "HG"), ("ATP", "H61"), ("HEM", "FE"), ("NAD", "C4N")]
+
+
+def gui_pt(R, ps, t, n_rep, n_snap, sps, swap_interval, ladder_ratio, max_angle, seed, exchange_seed):
+    """The GUI's replica-exchange loop (python/gui_main.py:1960-2013) over
+    the reference module's run_landscape_segment, slot k's engine seeded
+    with seed + k: slot 0's pooled snapshots and energies, exchange counts."""
+    rng = np.random.default_rng(exchange_seed)
+    ladder = [KT * ladder_ratio ** k for k in range(n_rep)]
+    engines = [R.PhysicsEngine() for _ in range(n_rep)]
+    for k, e in enumerate(engines):
+        R.seed(e, seed + k)
+    states, energies, cur = [ps] * n_rep, [None] * n_rep, [max_angle] * n_rep
+    snaps, ens, att, acc = [], [], 0, 0
+    for seg in range(max(1, n_snap // swap_interval)):
+        for k in range(n_rep):
+            sn, en, cm = engines[k].run_landscape_segment(states[k], t, swap_interval, sps, ladder[k], cur[k])
+            states[k], energies[k], cur[k] = sn[-1], en[-1], cm
+            if k == 0:
+                snaps.extend(sn)
+                ens.extend(en)
+        for a in range(seg % 2, n_rep - 1, 2):
+            b = a + 1
+            x = (energies[a] - energies[b]) * ((1.0 / ladder[a]) - (1.0 / ladder[b]))
+            att += 1
+            if rng.random() < (1.0 if x >= 0 else np.exp(x)):
+                states[a], states[b] = states[b], states[a]
+                energies[a], energies[b] = energies[b], energies[a]
+                cur[a], cur[b] = cur[b], cur[a]
+                acc += 1
+    xyz = np.array([[[p.x, p.y, p.z] for p in sn] for sn in snaps])
+    return xyz, np.array(ens), np.array([att, acc])
 
 
 def main():
@@ -98,6 +134,12 @@ def main():
                 out[name + "/diag_" + key] = np.array(v)
             out[name + "/diag_steps"] = np.array(DIAG[name][0])
             out[name + "/diag_seed"] = np.array(DIAG[name][1])
+        if name in PT:
+            xyz, e, sw = gui_pt(R, ps, t, *PT[name])
+            out[name + "/pt_args"] = np.array(PT[name], dtype=np.float64)
+            out[name + "/pt_xyz"] = xyz
+            out[name + "/pt_e"] = e
+            out[name + "/pt_swaps"] = sw
         print(name, len(ps), "atoms, total", out[name + "/potential"])
     np.savez_compressed(os.path.join(HERE, "pdb_golden.npz"), **out)
 

@@ -64,7 +64,7 @@ def test_large_trajectory_matches_oracle(ctx, name, make, mode, steps):
     c, _ = _topo(s)
     ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
     gpu = ctx.mc_run(mode, steps, n_chains=2, temperature=KT, max_angle=0.12, seed=4321, trace=True)
-    orc = O.mc_run(s, c, steps, KT, 0.12, seed=4321, mode=mode, trace=True, l1=True)
+    orc = O.mc_run(s, c, steps, KT, 0.12, seed=4321, mode=mode, trace=True, l1=True, null_floor=N.NULL_DE)
     gt, ot = gpu["trace"], orc["trace"]
     assert np.array_equal(gt["kind"], ot["kind"]), name
     assert np.array_equal(gt["accepted"], ot["accepted"]), name
@@ -75,7 +75,7 @@ def test_large_trajectory_matches_oracle(ctx, name, make, mode, steps):
     assert abs(gs["cur_e"] - os_["cur_e"]) <= 1e-6 * max(1.0, abs(os_["cur_e"])), name
     assert np.max(np.abs(gpu["xyz"][0] - orc["xyz"])) < 1e-8, name
     # the second chain draws from seed + 1
-    o2 = O.mc_run(s, c, steps, KT, 0.12, seed=4322, mode=mode)
+    o2 = O.mc_run(s, c, steps, KT, 0.12, seed=4322, mode=mode, null_floor=N.NULL_DE)
     assert np.max(np.abs(gpu["xyz"][1] - o2["xyz"])) < 1e-8, name
 
 
@@ -90,7 +90,7 @@ def test_complex_mc_segments_with_forces(ctx):
         ctx.set_positions(xyz)
         gpu = ctx.mc_run(N.MC_ENSEMBLE, 10, n_chains=1, temperature=KT, max_angle=0.12, seed=900 + seg, trace=True)
         cur = dict(s, xyz=xyz)
-        orc = O.mc_run(cur, c, 10, KT, 0.12, seed=900 + seg, mode=N.MC_ENSEMBLE, trace=True, l1=True)
+        orc = O.mc_run(cur, c, 10, KT, 0.12, seed=900 + seg, mode=N.MC_ENSEMBLE, trace=True, l1=True, null_floor=N.NULL_DE)
         assert np.array_equal(gpu["trace"]["accepted"], orc["trace"]["accepted"]), seg
         assert np.max(np.abs(gpu["xyz"][0] - orc["xyz"])) < 1e-8, seg
         xyz = gpu["xyz"][0]

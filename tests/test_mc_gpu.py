@@ -59,7 +59,7 @@ def test_trajectory_matches_oracle(ctx, mode, steps):
     c = S.csr(t)
     _load(ctx, s, c)
     gpu = ctx.mc_run(mode, steps, n_chains=1, temperature=KT, max_angle=0.12, seed=1234, trace=True)
-    orc = O.mc_run(s, c, steps, KT, 0.12, seed=1234, mode=mode, trace=True, l1=True)
+    orc = O.mc_run(s, c, steps, KT, 0.12, seed=1234, mode=mode, trace=True, l1=True, null_floor=N.NULL_DE)
     _compare(gpu, orc, steps, "mode%d" % mode)
 
 
@@ -70,7 +70,7 @@ def test_chains_use_consecutive_seeds(ctx):
     _load(ctx, s, c)
     gpu = ctx.mc_run(N.MC_ENSEMBLE, 150, n_chains=3, temperature=KT, seed=77)
     for k in range(3):
-        orc = O.mc_run(s, c, 150, KT, 0.12, seed=77 + k)
+        orc = O.mc_run(s, c, 150, KT, 0.12, seed=77 + k, null_floor=N.NULL_DE)
         assert gpu["stats"][k]["words"] == orc["stats"]["words"]
         assert np.max(np.abs(gpu["xyz"][k] - orc["xyz"])) < 1e-8
 
@@ -82,7 +82,7 @@ def test_disulfide_system_mc(ctx):
     c = S.csr(t)
     _load(ctx, s, c)
     gpu = ctx.mc_run(N.MC_ENSEMBLE, 200, temperature=KT, seed=9, trace=True)
-    orc = O.mc_run(s, c, 200, KT, 0.12, seed=9, trace=True, l1=True)
+    orc = O.mc_run(s, c, 200, KT, 0.12, seed=9, trace=True, l1=True, null_floor=N.NULL_DE)
     _compare(gpu, orc, 200, "ss")
 
 
@@ -93,7 +93,7 @@ def test_landscape_snapshots_fresh_energy(ctx):
     c = S.csr(t)
     _load(ctx, s, c)
     gpu = ctx.mc_run(N.MC_LANDSCAPE, 120, temperature=KT, seed=5, snapshot_every=40)
-    orc = O.mc_run(s, c, 120, KT, 0.12, seed=5, mode=O.MODE_LANDSCAPE, snapshot_every=40)
+    orc = O.mc_run(s, c, 120, KT, 0.12, seed=5, mode=O.MODE_LANDSCAPE, snapshot_every=40, null_floor=N.NULL_DE)
     assert np.max(np.abs(gpu["snapshots"][0] - orc["snapshots"])) < 1e-8
     rel = np.abs(gpu["energies"][0] - orc["energies"]) / np.abs(orc["energies"])
     assert rel.max() <= 1e-5
@@ -116,7 +116,7 @@ def test_dropin_generate_ensemble_seeded(ctx):
     eng = PP.PhysicsEngine()
     ens = eng.generate_ensemble(S.particles(s), t, 2, 100, KT, 0.12, seed=21)
     assert len(ens) == 2 and len(ens[0]) == len(s["xyz"])
-    orc = O.mc_run(s, S.csr(t), 100, KT, 0.12, seed=22)
+    orc = O.mc_run(s, S.csr(t), 100, KT, 0.12, seed=22, null_floor=N.NULL_DE)
     got = np.array([[p.x, p.y, p.z] for p in ens[1]])
     assert np.max(np.abs(got - orc["xyz"])) < 1e-8
 

@@ -186,3 +186,77 @@ def test_oracle_mc_diagnostic_matches_reference(golden, name):
     scale = sum(np.abs(ref[k]) for k in DIAG_KEYS[3:]) + 1e-12
     for k in DIAG_KEYS[3:]:
         assert np.all(np.abs(got[k] - ref[k]) <= 1e-9 * scale), (name, k)
+
+
+def test_exchange_generator_is_numpys():
+    """The replica-exchange draws come from numpy.random.default_rng(seed)
+    .random() (python/gui_main.py:1961, 2009): the restated SeedSequence +
+    PCG64 stream equals numpy's bit for bit, one- and two-word seeds."""
+    for seed in (0, 1, 3, 12345, 2**32 - 1, 2**32 + 7, 2**63 + 11):
+        assert np.array_equal(O.pcg64_random(seed, 64), np.random.default_rng(seed).random(64)), seed
+
+
+def test_oracle_landscape_pt_matches_reference(golden):
+    """The oracle's PT driver against the GUI's loop over the reference
+    module's run_landscape_segment (seeded engines; tests/golden): pooled
+    snapshots bit for bit, energies to 1e-12, exchange counts equal."""
+    import protein_physics as PP
+    s = pdb.load(_path("SM_P01308"))
+    c = O.topo_to_csr(pdb.topology(s, PP))
+    R, ns, sps, si, lr, ma, seed, ex = golden["SM_P01308/pt_args"].tolist()
+    o = O.landscape_pt(s, c, int(R), int(ns), int(sps), 0.0019872041 * 300.0, lr, int(si), ma, seed=int(seed),
+                       exchange_seed=int(ex))
+    assert np.array_equal(o["swaps"], golden["SM_P01308/pt_swaps"])
+    assert np.array_equal(o["snapshots"], golden["SM_P01308/pt_xyz"])
+    ref = golden["SM_P01308/pt_e"]
+    assert np.all(np.abs(o["energies"] - ref) <= 1e-12 * np.abs(ref))
+
+
+def test_null_move_rule():
+    """ALMA_NULL_DE (include/alma.h) and the oracle's null_floor: with the
+    floor, a move with |dE| <= 1e-10 takes the Metropolis draw.  Against the
+    reference's rule (floor 0) the two runs agree step for step up to the
+    first null move whose rounding-noise dE is negative (there the
+    reference accepts without a draw): same move, both accepted, and the
+    generator streams part from the next step on."""
+    import re
+    from alma import _native as N
+    hdr = open(os.path.join(os.path.dirname(GOLDEN), "..", "include", "alma.h")).read()
+    assert float(re.search(r"#define ALMA_NULL_DE (\S+)", hdr).group(1)) == N.NULL_DE
+    import protein_physics as PP
+    s = pdb.load(_path("SM_P01308"))
+    c = O.topo_to_csr(pdb.topology(s, PP))
+    kt = 0.0019872041 * 300.0
+    ref = O.mc_run(s, c, 1500, kt, 0.12, seed=7, mode=O.MODE_LANDSCAPE, trace=True)["trace"]
+    flo = O.mc_run(s, c, 1500, kt, 0.12, seed=7, mode=O.MODE_LANDSCAPE, trace=True, null_floor=N.NULL_DE)["trace"]
+    null = np.abs(ref["dE"]) <= N.NULL_DE
+    neg = np.flatnonzero(null & (ref["dE"] < 0))
+    assert len(neg) > 0 and np.sum(null & (ref["dE"] >= 0)) > 0
+    k = int(neg[0])
+    for key in ("kind", "bond", "delta", "accepted", "words_before"):
+        assert np.array_equal(ref[key][:k + 1], flo[key][:k + 1]), key
+    assert ref["accepted"][k] == 1
+    assert flo["words_before"][k + 1] == ref["words_before"][k + 1] + 2   # the floor's draw: 2 words
+    # every null move is accepted under either rule
+    assert np.all(ref["accepted"][null] == 1)
+
+
+def test_oracle_landscape_pt_rules():
+    """The oracle's PT driver: one replica equals a plain landscape
+    trajectory of the same length (same generator, a fresh build per
+    segment, step threaded); exchange counts follow the alternating-pair
+    schedule; pooled energies are fresh calculate_potential values."""
+    import protein_physics as PP
+    s = pdb.load(_path("SM_P01308"))
+    c = O.topo_to_csr(pdb.topology(s, PP))
+    kt = 0.0019872041 * 300.0
+    r = O.landscape_pt(s, c, 3, 6, 15, kt, 1.6, 3, 0.12, seed=5, exchange_seed=2)
+    assert r["swaps"][0] == 1 + 1          # segment 0: pair (0,1); segment 1: pair (1,2)
+    assert len(r["energies"]) == 6
+    last = dict(s, xyz=r["snapshots"][-1])
+    assert abs(O.energy_terms(last, c)["TOTAL"] - r["energies"][-1]) <= 1e-12 * abs(r["energies"][-1])
+    one = O.landscape_pt(s, c, 1, 2, 15, kt, 1.6, 2, 0.12, seed=9, exchange_seed=0)
+    traj = O.mc_run(s, c, 30, kt, 0.12, seed=9, mode=O.MODE_LANDSCAPE, snapshot_every=15)
+    assert np.array_equal(one["snapshots"], traj["snapshots"])
+    # (the oracle's energy sums run over OpenMP threads: order not fixed)
+    assert np.allclose(one["energies"], traj["energies"], rtol=1e-12, atol=0)

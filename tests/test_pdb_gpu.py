@@ -80,13 +80,13 @@ def test_mc_trajectory_matches_oracle(ctx, name, mode):
     ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
     steps = 60
     gpu = ctx.mc_run(mode, steps, n_chains=2, temperature=KT, max_angle=0.12, seed=2024, trace=True)
-    orc = O.mc_run(s, c, steps, KT, 0.12, seed=2024, mode=mode, trace=True, l1=True)
+    orc = O.mc_run(s, c, steps, KT, 0.12, seed=2024, mode=mode, trace=True, l1=True, null_floor=N.NULL_DE)
     gt, ot = gpu["trace"], orc["trace"]
     assert np.array_equal(gt["kind"], ot["kind"]), name
     assert np.array_equal(gt["accepted"], ot["accepted"]), name
     assert np.all(np.abs(gt["dE"] - ot["dE"]) <= 1e-5 * ot["dE_l1"] + 1e-9), name
     assert np.max(np.abs(gpu["xyz"][0] - orc["xyz"])) < 1e-8, name
-    o2 = O.mc_run(s, c, steps, KT, 0.12, seed=2025, mode=mode)
+    o2 = O.mc_run(s, c, steps, KT, 0.12, seed=2025, mode=mode, null_floor=N.NULL_DE)
     assert np.max(np.abs(gpu["xyz"][1] - o2["xyz"])) < 1e-8, name
 
 
@@ -102,7 +102,7 @@ def test_mc_diagnostic_matches_oracle(ctx, name, steps, seed):
     s, t, c = _parsed(name)
     ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
     got = ctx.mc_diagnostic(steps, temperature=KT, max_angle=0.12, seed=seed)
-    orc = O.mc_diagnostic(s, c, steps, KT, 0.12, seed=seed, l1=True)
+    orc = O.mc_diagnostic(s, c, steps, KT, 0.12, seed=seed, l1=True, null_floor=N.NULL_DE)
     assert np.array_equal(got["move_type"], orc["move_type"]), name
     assert np.array_equal(got["accepted"], orc["accepted"]), name
     assert np.array_equal(got["proposed_delta"], orc["proposed_delta"]), name
@@ -123,6 +123,33 @@ def test_dropin_run_mc_diagnostic(golden):
     assert np.array_equal(np.array(rec[1]), ref[1])
     # error scale of each move: sum of |per-item change| (from the oracle's
     # replay of the same moves)
-    tol = 1e-5 * O.mc_diagnostic(s, c, 200, KT, 0.12, seed=57, l1=True)["dE_l1"] + 1e-9
+    tol = 1e-5 * O.mc_diagnostic(s, c, 200, KT, 0.12, seed=57, l1=True, null_floor=N.NULL_DE)["dE_l1"] + 1e-9
     for got, want in zip(rec[3:], ref[3:]):
         assert np.all(np.abs(np.array(got) - want) <= tol)
+
+
+def test_landscape_pt_matches_oracle(ctx):
+    """The GUI's replica-exchange driver on the device against the oracle's:
+    4 slots, 2 segments of 3 snapshots x 15 steps on insulin (concerted,
+    crankshaft and torsion moves, disulfides): same pooled snapshots
+    (1e-8 A), fresh energies (1e-5 relative) and exchange counts."""
+    s, t, c = _parsed("SM_P01308")
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    g = ctx.landscape_pt(4, 6, 15, temperature=KT, ladder_ratio=1.6, swap_interval=3, max_angle=0.12, seed=21,
+                         exchange_seed=4)
+    o = O.landscape_pt(s, c, 4, 6, 15, KT, 1.6, 3, 0.12, seed=21, exchange_seed=4, null_floor=N.NULL_DE)
+    assert np.array_equal(g["swaps"], o["swaps"])
+    assert np.max(np.abs(g["snapshots"] - o["snapshots"])) < 1e-8
+    assert np.all(np.abs(g["energies"] - o["energies"]) <= 1e-5 * np.abs(o["energies"]))
+
+
+def test_dropin_run_parallel_tempering():
+    import protein_physics as PP
+    s, t, c = _parsed("SM_P01308")
+    snaps, es = PP.PhysicsEngine().run_parallel_tempering(pdb.particles(s, PP), t, 6, 15, KT, 0.12, 4, 1.6, 3,
+                                                          rng_seed=4, seed=21)
+    o = O.landscape_pt(s, c, 4, 6, 15, KT, 1.6, 3, 0.12, seed=21, exchange_seed=4, null_floor=N.NULL_DE)
+    assert len(snaps) == 6 and len(es) == 6
+    x = np.array([[p.x, p.y, p.z] for p in snaps[-1]])
+    assert np.max(np.abs(x - o["snapshots"][-1])) < 1e-8
+    assert np.all(np.abs(np.array(es) - o["energies"]) <= 1e-5 * np.abs(o["energies"]))
