@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the runtime's copy kernel), "
                          "the reference for the FETCH_SIZE / WRITE_SIZE counter scale")
+    ap.add_argument("--ana-ca", type=int, default=500,
+                    help="ensemble metrics: Calpha atoms (0 = skip)")
+    ap.add_argument("--ana-snaps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -81,6 +84,9 @@ def main():
         return
 
     import numpy as np
+    # torch before the library: one HIP runtime in the process (torch's), so
+    # torch device tensors handed to the library are visible to it
+    import torch  # noqa: F401
     import protein_physics as PP
     from alma import _native as N
     from alma import synth
@@ -169,6 +175,11 @@ def main():
         cx = _complex_rate(s, args, PP, kt)
         log("complex: %.1f MC steps/s with a force evaluation every 10 steps; %.1f dE evals/s over 64 chains"
             % (cx["segment_mc_steps_per_s"], cx["chains64_evals_per_s"]))
+    ana = None
+    if args.ana_ca > 0:
+        ana = _analysis_rate(ctx, args)
+        log("ensemble metrics: kernel %.3f ms, %.3g pair-snapshots/s" % (ana["kernel_ms"],
+                                                                          ana["pair_snapshots_per_s"]))
     # the energy context is reused by the later calls: reload the bench system
     ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], csr)
 
@@ -199,6 +210,7 @@ def main():
         "mala": mala,
         "parallel_tempering": pt,
         "large_complex": cx,
+        "ensemble_analysis": ana,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -267,6 +279,71 @@ def _mala_rate(ctx, args, kt):
     return {"steps": args.mala_steps, "steps_per_s": args.mala_steps / dt,
             "accept_rate": st["accepted"] / max(st["moves"], 1), "mala_step_rad": 0.02,
             "note": "one system; each step = proposal + fresh list + energy + gradient (force evaluation)"}
+
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64
+ANA_OPS = 11              # per pair and snapshot: 3 sub, 3 mul, 2 add, sqrt, sum, compare
+
+
+def _analysis_rate(ctx, args):
+    """protein_analysis's two ensemble matrices (compute_mean_dist_matrix +
+    compute_contact_freq_matrix, one fused kernel) of a seeded random-walk
+    ensemble of --ana-ca Calpha x --ana-snaps snapshots: device kernel time
+    (HIP events), the call with coordinates already in device memory, the
+    call from host arrays (PCIe both ways), and the reference module's own
+    two calls (its setup.py build, OpenMP on the host cores) on the same
+    array."""
+    import numpy as np
+    import torch
+    n, S = args.ana_ca, args.ana_snaps
+    rng = np.random.default_rng(99)
+    x = np.cumsum(rng.normal(size=(S, n, 3)) * 2.3, axis=1)
+    ctx.ensemble_matrices(x)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m, f = ctx.ensemble_matrices(x)
+    host_ms = (time.perf_counter() - t0) / reps * 1e3
+    d = torch.tensor(x, device="cuda:0")
+    dm = torch.empty((n, n), dtype=torch.float64, device="cuda:0")
+    df = torch.empty((n, n), dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    ctx.ensemble_matrices_device(d.data_ptr(), S, n, 8.0, dm.data_ptr(), df.data_ptr())
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.ensemble_matrices_device(d.data_ptr(), S, n, 8.0, dm.data_ptr(), df.data_ptr())
+        kms.append(ctx.ensemble_timing()["kernel_ms"])
+    dev_ms = (time.perf_counter() - t0) / reps * 1e3
+    same = bool(np.array_equal(dm.cpu().numpy(), m) and np.array_equal(df.cpu().numpy(), f))
+    kernel_ms = float(np.mean(kms))
+    units = S * n * (n - 1) / 2
+    ops = ANA_OPS * units / (kernel_ms * 1e-3) / 1e12
+    out = {"workload": "mean-distance + contact-frequency matrices, %d Calpha x %d snapshots (random walk)" % (n, S),
+           "n_ca": n, "n_snaps": S, "kernel_ms": kernel_ms, "device_call_ms": dev_ms, "host_call_ms": host_ms,
+           "pair_snapshots_per_s": units / (kernel_ms * 1e-3),
+           "roofline": {"bound": "fp64 valu", "achieved": ops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": ops / FP64_PEAK_TFLOPS, "ops_per_pair_snapshot": ANA_OPS,
+                        "kernel": "k_ensemble", "tiles": ctx.ensemble_timing()["tiles"]},
+           "device_equals_host_entry": same}
+    # the reference module (its setup.py build) on the host cores
+    import glob
+    import importlib.util
+    hits = glob.glob(os.path.join(ROOT, "oracle", "_ref", "protein_analysis*.so"))
+    if hits and not args.no_cpu:
+        spec = importlib.util.spec_from_file_location("protein_analysis", hits[0])
+        ref = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(ref)
+        t0 = time.perf_counter()
+        rm = ref.compute_mean_dist_matrix(x)
+        rf = ref.compute_contact_freq_matrix(x, 8.0)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        out["cpu_reference"] = {"ms": cpu_ms, "kind": "reference",
+                                "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                                "max_rel_diff_mean": float(np.max(np.abs(rm - m) / np.maximum(np.abs(rm), 1e-300))),
+                                "freq_equal": bool(np.array_equal(rf, f)),
+                                "speedup_kernel": cpu_ms / kernel_ms, "speedup_host_call": cpu_ms / host_ms}
+    return out
 
 
 def _segment(s, n_res):

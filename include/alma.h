@@ -257,6 +257,29 @@ alma_status alma_pt_energy(alma_ctx* ctx, double* terms);
  * cell table is rolled back and run again with larger ones)}. */
 alma_status alma_pt_timing(alma_ctx* ctx, double* out);
 
+/* ---- ensemble metrics (protein_analysis) ---- */
+/* compute_mean_dist_matrix and compute_contact_freq_matrix
+ * (src/analysis_ext.cpp:85-167, module def :364-392): coords is
+ * (n_snaps, n_ca, 3) float64, C order; mean_dist / contact_freq (either may
+ * be NULL, not both) receive (n_ca, n_ca) float64 matrices: per residue pair
+ * the sum over snapshots of |r_i - r_j| (resp. the count of snapshots with
+ * |r_i - r_j|^2 < cutoff^2) times 1 / n_snaps, symmetric, diagonal 0; all
+ * zeros when n_snaps or n_ca is 0.  Each pair's sum runs over the snapshots
+ * in order, as the reference's loop, so results equal it bit for bit.
+ * Host pointers (staged through device buffers held by the context). */
+alma_status alma_ensemble_matrices(alma_ctx* ctx, const double* coords, int32_t n_snaps, int32_t n_ca,
+                                   double cutoff, double* mean_dist, double* contact_freq);
+/* The same over device pointers (memory of the context's device, written on
+ * the context's stream; the call returns when the matrices are complete and
+ * waits for no other stream).  No counterpart in the reference: for callers
+ * whose snapshots are already in device memory. */
+alma_status alma_ensemble_matrices_device(alma_ctx* ctx, const double* coords, int32_t n_snaps, int32_t n_ca,
+                                          double cutoff, double* mean_dist, double* contact_freq);
+/* out[2] = {device ms of the last ensemble kernel (HIP events on the
+ * context stream), workgroups (tiles of 4 rows x 64 columns)}; no reference
+ * counterpart. */
+alma_status alma_ensemble_timing(alma_ctx* ctx, double* out);
+
 #ifdef __cplusplus
 }
 #endif

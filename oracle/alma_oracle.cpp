@@ -1290,6 +1290,42 @@ void orc_pcg64_random(uint64_t seed, int n, double* out) {
     for (int k = 0; k < n; ++k) out[k] = g.random();
 }
 
+// compute_mean_dist_matrix (src/analysis_ext.cpp:85-123) and
+// compute_contact_freq_matrix (:129-167): for each snapshot in order, every
+// pair i < j adds its distance (resp. 1.0 when distance^2 < cutoff^2) to
+// both (i, j) and (j, i); then every element is multiplied by 1 / n_snaps.
+void orc_ensemble_matrices(int n_snaps, int n, const double* xyz, double cutoff, double* mean, double* freq) {
+    const size_t m = (size_t)n * n;
+    if (mean) std::fill(mean, mean + m, 0.0);
+    if (freq) std::fill(freq, freq + m, 0.0);
+    if (n_snaps == 0 || n == 0) return;
+    const double cut2 = cutoff * cutoff;
+    for (int s = 0; s < n_snaps; ++s) {
+        const double* p = xyz + (size_t)s * n * 3;
+#pragma omp parallel for schedule(static) if (n > 100)
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                const double dx = p[3 * i] - p[3 * j], dy = p[3 * i + 1] - p[3 * j + 1],
+                             dz = p[3 * i + 2] - p[3 * j + 2];
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (mean) {
+                    const double d = std::sqrt(d2);
+                    mean[(size_t)i * n + j] += d;
+                    mean[(size_t)j * n + i] += d;
+                }
+                if (freq && d2 < cut2) {
+                    freq[(size_t)i * n + j] += 1.0;
+                    freq[(size_t)j * n + i] += 1.0;
+                }
+            }
+    }
+    const double inv = 1.0 / (double)n_snaps;
+    for (size_t k = 0; k < m; ++k) {
+        if (mean) mean[k] *= inv;
+        if (freq) freq[k] *= inv;
+    }
+}
+
 // _PTRunnable._run_pt, python/gui_main.py:1960-2013, with each slot's
 // run_landscape_segment (src/physics_engine.cpp:2952-3270) as a chain that
 // starts from the slot's configuration and step and continues the slot's

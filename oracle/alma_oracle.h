@@ -210,6 +210,12 @@ int orc_landscape_pt(int n, const double* xyz0, const double* q, const double* r
  * XSL-RR, (next64 >> 11) * 2^-53) */
 void orc_pcg64_random(uint64_t seed, int n, double* out);
 
+/* compute_mean_dist_matrix / compute_contact_freq_matrix
+ * (src/analysis_ext.cpp:85-167): (n, n) matrices of (n_snaps, n, 3)
+ * coordinates, each element summed over the snapshots in order; either
+ * output may be NULL. */
+void orc_ensemble_matrices(int n_snaps, int n, const double* xyz, double cutoff, double* mean, double* freq);
+
 #ifdef __cplusplus
 }
 #endif

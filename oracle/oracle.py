@@ -436,6 +436,20 @@ def landscape_pt(sysd, topo_csr, n_replicas, n_snapshots, steps_per_snap, temper
     return {"snapshots": px, "energies": pe, "swaps": sw}
 
 
+def ensemble_matrices(coords, cutoff=8.0):
+    """compute_mean_dist_matrix and compute_contact_freq_matrix
+    (src/analysis_ext.cpp:85-167) of (n_snaps, n_ca, 3) coordinates."""
+    x = np.ascontiguousarray(coords, dtype=np.float64)
+    if x.ndim != 3 or x.shape[2] != 3:
+        raise ValueError("coords must be a 3-D array (n_snaps, n_ca, 3)")
+    n = x.shape[1]
+    m, f = np.zeros((n, n)), np.zeros((n, n))
+    L = lib()
+    L.orc_ensemble_matrices.argtypes = [C.c_int, C.c_int, _D, C.c_double, _D, _D]
+    L.orc_ensemble_matrices(x.shape[0], n, _p(x, C.c_double), float(cutoff), _p(m, C.c_double), _p(f, C.c_double))
+    return m, f
+
+
 def pcg64_random(seed, n):
     """numpy.random.default_rng(seed).random(n), restated."""
     out = np.zeros(n)

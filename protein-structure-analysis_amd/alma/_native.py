@@ -112,6 +112,10 @@ def lib():
         L.alma_pt_trace.argtypes = [P, _I, _D, _D, _I]
         L.alma_pt_energy.argtypes = [P, _D]
         L.alma_pt_timing.argtypes = [P, _D]
+        L.alma_ensemble_matrices.argtypes = [P, _D, C.c_int32, C.c_int32, C.c_double, _D, _D]
+        L.alma_ensemble_matrices_device.argtypes = [P, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_void_p,
+                                                    C.c_void_p]
+        L.alma_ensemble_timing.argtypes = [P, _D]
         _LIB = L
     return _LIB
 
@@ -120,7 +124,8 @@ EXPORTED = ("alma_create", "alma_destroy", "alma_last_error", "alma_device_name"
             "alma_set_system", "alma_set_positions", "alma_energy", "alma_nlist",
             "alma_born_radii", "alma_mc_run", "alma_mc_trace", "alma_mc_timing", "alma_mc_diagnostic",
             "alma_landscape_pt", "alma_forces",
-            "alma_pt_set", "alma_pt_run", "alma_pt_trace", "alma_pt_energy", "alma_pt_timing")
+            "alma_pt_set", "alma_pt_run", "alma_pt_trace", "alma_pt_energy", "alma_pt_timing",
+            "alma_ensemble_matrices", "alma_ensemble_matrices_device", "alma_ensemble_timing")
 
 
 def _p(a, t):
@@ -291,6 +296,34 @@ class Context:
         self._check(lib().alma_landscape_pt(self._h, C.byref(p), _p(px, C.c_double), _p(pe, C.c_double),
                                             _p(cnt, C.c_int32), sw.ctypes.data_as(C.POINTER(C.c_int64))))
         return {"snapshots": px[:int(cnt[0])], "energies": pe[:int(cnt[0])], "swaps": sw}
+
+    def ensemble_matrices(self, coords, cutoff: float = 8.0, mean: bool = True, freq: bool = True):
+        """compute_mean_dist_matrix / compute_contact_freq_matrix
+        (src/analysis_ext.cpp:85-167) of (n_snaps, n_ca, 3) coordinates on
+        the device: (mean_dist, contact_freq), either None when not asked."""
+        x = np.ascontiguousarray(coords, dtype=np.float64)
+        if x.ndim != 3 or x.shape[2] != 3:
+            raise ValueError("coords must be a 3-D array (n_snaps, n_ca, 3)")
+        s, n = x.shape[0], x.shape[1]
+        m = np.empty((n, n)) if mean else None
+        f = np.empty((n, n)) if freq else None
+        self._check(lib().alma_ensemble_matrices(self._h, _p(x, C.c_double), s, n, float(cutoff),
+                                                 _p(m, C.c_double) if mean else None,
+                                                 _p(f, C.c_double) if freq else None))
+        return m, f
+
+    def ensemble_matrices_device(self, coords_ptr: int, n_snaps: int, n_ca: int, cutoff: float, mean_ptr: int,
+                                 freq_ptr: int):
+        """The same over device pointers (ints, e.g. torch's data_ptr(); 0
+        for an output not wanted)."""
+        self._check(lib().alma_ensemble_matrices_device(self._h, C.c_void_p(coords_ptr), n_snaps, n_ca,
+                                                        float(cutoff), C.c_void_p(mean_ptr or None),
+                                                        C.c_void_p(freq_ptr or None)))
+
+    def ensemble_timing(self) -> dict:
+        out = np.zeros(2)
+        self._check(lib().alma_ensemble_timing(self._h, _p(out, C.c_double)))
+        return {"kernel_ms": float(out[0]), "tiles": int(out[1])}
 
     def mc_timing(self) -> dict:
         out = np.zeros(12)

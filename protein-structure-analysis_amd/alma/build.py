@@ -3,6 +3,7 @@
   libalma.so            gfx950 HIP kernels + the C ABI (include/alma.h), hipcc
   protein_physics*.so   the drop-in pybind11 module over the C ABI, g++
   protein_physics_cuda*.so  the same module under the reference GPU module's name
+  protein_analysis*.so  the drop-in ensemble-metrics / knot-helper module, g++
   oracle/liboracle.so   (+ oracle/_ref when the reference sources exist):
                         test-infrastructure builds, via oracle/Makefile
 
@@ -21,8 +22,9 @@ LIB = os.path.join(PKG, "libalma.so")
 EXT = sysconfig.get_config_var("EXT_SUFFIX")
 MODULE = os.path.join(PKG, "protein_physics" + EXT)
 MODULE_GPU_NAME = os.path.join(PKG, "protein_physics_cuda" + EXT)
+MODULE_ANALYSIS = os.path.join(PKG, "protein_analysis" + EXT)
 
-HIP_SOURCES = ["energy.hip", "mc.hip", "forces.hip", "batch.hip", "alma_abi.cpp"]
+HIP_SOURCES = ["energy.hip", "mc.hip", "forces.hip", "batch.hip", "analysis.hip", "alma_abi.cpp"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                "-munsafe-fp-atomics", "-Wno-unused-result"]
@@ -49,11 +51,14 @@ def build_module():
             "-I", inc, "-I", pybind11.get_include(), "-I", os.path.join(ROOT, "include"),
             os.path.join(CSRC, "protein_physics.cpp"), os.path.join(CSRC, "topology.cpp"),
             "-L", PKG, "-lalma", "-Wl,-rpath,$ORIGIN"]
-    jobs = []
-    for out, name in ((MODULE, "protein_physics"), (MODULE_GPU_NAME, "protein_physics_cuda")):
-        cmd = base + ["-DALMA_PY_MODULE=" + name, "-o", out + ".tmp"]
-        jobs.append((out, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                                                text=True)))
+    cmds = [(out, base + ["-DALMA_PY_MODULE=" + name, "-o", out + ".tmp"])
+            for out, name in ((MODULE, "protein_physics"), (MODULE_GPU_NAME, "protein_physics_cuda"))]
+    cmds.append((MODULE_ANALYSIS, ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", inc,
+                                   "-I", pybind11.get_include(), "-I", os.path.join(ROOT, "include"),
+                                   os.path.join(CSRC, "protein_analysis.cpp"), "-L", PKG, "-lalma",
+                                   "-Wl,-rpath,$ORIGIN", "-o", MODULE_ANALYSIS + ".tmp"]))
+    jobs = [(out, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
+            for out, cmd in cmds]
     for out, cmd, pr in jobs:
         log = pr.communicate()[0]
         if pr.returncode != 0:

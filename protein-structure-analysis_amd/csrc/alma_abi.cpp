@@ -82,6 +82,86 @@ void free_topo(DevTopo& t) {
     t = DevTopo{};
 }
 
+void free_analysis(alma_ctx* ctx) {
+    dev_free(ctx->an_xyz);
+    dev_free(ctx->an_mean);
+    dev_free(ctx->an_freq);
+    dev_free(ctx->an_tiles);
+    ctx->an_xyz_cap = ctx->an_mat_cap = 0;
+    ctx->an_tiles_n = -1;
+    ctx->an_n_tiles = 0;
+}
+
+// Ensemble-mean distance and contact-frequency matrices
+// (src/analysis_ext.cpp:85-167) of (n_snaps, n, 3) coordinates.  dev: the
+// coordinate and matrix pointers are device memory (else host memory,
+// staged through the context's buffers).  mean / freq may be null, not both.
+void ensemble(alma_ctx* ctx, const double* coords, int n_snaps, int n, double cutoff, double* mean, double* freq,
+              bool dev) {
+    if (n_snaps < 0 || n < 0) throw Fail{ALMA_EINVAL, "coords must be a 3-D array (n_snaps, n_ca, 3)"};
+    if (!mean && !freq) throw Fail{ALMA_EINVAL, "no output matrix"};
+    if (n_snaps > 0 && n > 0 && !coords) throw Fail{ALMA_EINVAL, "coords missing"};
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t mat = (size_t)n * n;
+    ctx->t_an_ms = 0.0;
+    if (n_snaps == 0 || n == 0) {
+        // no snapshot: zero matrices, as the reference returns them
+        for (double* m : {mean, freq}) {
+            if (!m || mat == 0) continue;
+            if (dev)
+                HIP_TRY(hipMemsetAsync(m, 0, sizeof(double) * mat, ctx->stream));
+            else
+                std::fill(m, m + mat, 0.0);
+        }
+        if (dev) HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return;
+    }
+    if (ctx->an_tiles_n != n) {
+        std::vector<int2> t;
+        ensemble_tiles(n, t);
+        dev_upload(&ctx->an_tiles, t.data(), t.size());
+        ctx->an_tiles_n = n;
+        ctx->an_n_tiles = (int)t.size();
+    }
+    const double* d_xyz = coords;
+    double* d_mean = mean;
+    double* d_freq = freq;
+    const size_t nx = (size_t)n_snaps * n * 3;
+    if (!dev) {
+        if (nx > ctx->an_xyz_cap) {
+            dev_alloc(&ctx->an_xyz, nx);
+            ctx->an_xyz_cap = nx;
+        }
+        if (mat > ctx->an_mat_cap) {
+            dev_alloc(&ctx->an_mean, mat);
+            dev_alloc(&ctx->an_freq, mat);
+            ctx->an_mat_cap = mat;
+        }
+        HIP_TRY(hipMemcpyAsync(ctx->an_xyz, coords, sizeof(double) * nx, hipMemcpyHostToDevice, ctx->stream));
+        d_xyz = ctx->an_xyz;
+        d_mean = mean ? ctx->an_mean : nullptr;
+        d_freq = freq ? ctx->an_freq : nullptr;
+    }
+    if (!ctx->ev0) {
+        HIP_TRY(hipEventCreate(&ctx->ev0));
+        HIP_TRY(hipEventCreate(&ctx->ev1));
+    }
+    HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(launch_ensemble(d_xyz, n_snaps, n, cutoff * cutoff, ctx->an_tiles, ctx->an_n_tiles, d_mean, d_freq,
+                            ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    if (!dev) {
+        if (mean)
+            HIP_TRY(hipMemcpyAsync(mean, d_mean, sizeof(double) * mat, hipMemcpyDeviceToHost, ctx->stream));
+        if (freq)
+            HIP_TRY(hipMemcpyAsync(freq, d_freq, sizeof(double) * mat, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->t_an_ms = ms;
+}
+
 void free_forces(alma_ctx* ctx) {
     dev_free(ctx->f_grad);
     dev_free(ctx->f_dEda);
@@ -1569,6 +1649,7 @@ void alma_destroy(alma_ctx* ctx) {
     free_chains(ctx->chains);
     free_forces(ctx);
     free_batch(ctx);
+    free_analysis(ctx);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -1790,6 +1871,24 @@ alma_status alma_landscape_pt(alma_ctx* ctx, const alma_landscape_pt_params* p, 
         std::vector<alma_mc_stats> st(p->n_replicas);
         run_mc(ctx, mp, xyz.data(), st.data(), nullptr, nullptr, &pt);
         if (n_pool) *n_pool = pt.n_seg * p->swap_interval;
+    });
+}
+
+alma_status alma_ensemble_matrices(alma_ctx* ctx, const double* coords, int32_t n_snaps, int32_t n_ca,
+                                   double cutoff, double* mean_dist, double* contact_freq) {
+    return guarded(ctx, [&] { ensemble(ctx, coords, n_snaps, n_ca, cutoff, mean_dist, contact_freq, false); });
+}
+
+alma_status alma_ensemble_matrices_device(alma_ctx* ctx, const double* coords, int32_t n_snaps, int32_t n_ca,
+                                          double cutoff, double* mean_dist, double* contact_freq) {
+    return guarded(ctx, [&] { ensemble(ctx, coords, n_snaps, n_ca, cutoff, mean_dist, contact_freq, true); });
+}
+
+alma_status alma_ensemble_timing(alma_ctx* ctx, double* out) {
+    return guarded(ctx, [&] {
+        if (!out) throw Fail{ALMA_EINVAL, "output buffer missing"};
+        out[0] = ctx->t_an_ms;
+        out[1] = (double)ctx->an_n_tiles;
     });
 }
 

@@ -290,6 +290,11 @@ hipError_t launch_batch_pair(const Batch& b, int lanes, int block, hipStream_t s
 hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int nlist, bool diag, hipStream_t st);
 // curE of every chain: total energy at its last list build + accepted dE since
 hipError_t launch_chain_energy(ChainBatch& b, const DevSys& s, hipStream_t st);
+// ensemble metrics (analysis.hip): tiles of the (n x n) matrices, and the
+// kernel over (n_snaps, n, 3) coordinates; mean / freq may be null
+void ensemble_tiles(int n, std::vector<int2>& tiles);
+hipError_t launch_ensemble(const double* xyz, int n_snaps, int n, double cut2, const int2* tiles, int n_tiles,
+                           double* mean, double* freq, hipStream_t st);
 
 // ---- kernels / launchers (energy.hip) ----
 hipError_t launch_build_nl(DevSys& s, hipStream_t st);
@@ -348,6 +353,13 @@ struct alma_ctx {
     // forces buffers (sized for sys.n / n_rot when allocated)
     double *f_grad = nullptr, *f_dEda = nullptr, *f_pre = nullptr, *f_dEdt = nullptr;
     int f_n = -1;
+    // ensemble metrics: device buffers of the host-pointer entry (grown on
+    // demand), the tile list of the last matrix size, last kernel time
+    double *an_xyz = nullptr, *an_mean = nullptr, *an_freq = nullptr;
+    size_t an_xyz_cap = 0, an_mat_cap = 0;
+    int2* an_tiles = nullptr;
+    int an_tiles_n = -1, an_n_tiles = 0;
+    double t_an_ms = 0.0;
     // MC trace of the last run
     std::vector<int> tr_kind, tr_bond, tr_acc;
     std::vector<double> tr_delta, tr_dE, tr_parts, tr_hc;
