@@ -179,6 +179,21 @@ typedef struct {
     uint32_t seed;
     uint32_t pad;
     uint64_t exchange_seed;
+    /* Optional ensemble metrics of the pool, computed on the device from the
+     * snapshots as they are taken (no host round trip): when n_analysis > 0,
+     * atoms analysis_atoms[0..n_analysis) (e.g. the Calpha indices) of every
+     * pooled snapshot are gathered in device memory, and mean_dist /
+     * contact_freq (n_analysis^2 each, either may be NULL) receive
+     * compute_mean_dist_matrix / compute_contact_freq_matrix
+     * (src/analysis_ext.cpp:85-167, cutoff contact_cutoff) over the pool, as
+     * gui_main.py:1107-1109 / 1197-1199 compute them from the pooled
+     * snapshots.  No counterpart in the reference's C++ API. */
+    const int32_t* analysis_atoms;
+    int32_t n_analysis;
+    int32_t pad2;
+    double contact_cutoff;
+    double* mean_dist;
+    double* contact_freq;
 } alma_landscape_pt_params;
 
 alma_status alma_landscape_pt(alma_ctx* ctx, const alma_landscape_pt_params* p, double* pool_xyz, double* pool_e,

@@ -51,7 +51,8 @@ class LptParams(C.Structure):
     _fields_ = [("n_replicas", C.c_int32), ("n_snapshots", C.c_int32), ("steps_per_snap", C.c_int32),
                 ("swap_interval", C.c_int32), ("temperature", C.c_double), ("ladder_ratio", C.c_double),
                 ("max_angle", C.c_double), ("seed", C.c_uint32), ("pad", C.c_uint32),
-                ("exchange_seed", C.c_uint64)]
+                ("exchange_seed", C.c_uint64), ("analysis_atoms", _I), ("n_analysis", C.c_int32),
+                ("pad2", C.c_int32), ("contact_cutoff", C.c_double), ("mean_dist", _D), ("contact_freq", _D)]
 
 
 class PtSystems(C.Structure):
@@ -281,10 +282,14 @@ class Context:
         return {k: v[:int(mc[0])] for k, v in a.items()}
 
     def landscape_pt(self, n_replicas, n_snapshots, steps_per_snap, temperature=0.6, ladder_ratio=1.6,
-                     swap_interval=5, max_angle=0.12, seed=5489, exchange_seed=0) -> dict:
+                     swap_interval=5, max_angle=0.12, seed=5489, exchange_seed=0, analysis_atoms=None,
+                     contact_cutoff=8.0) -> dict:
         """The GUI's replica-exchange driver over landscape segments, all
         slots on the device: slot 0's pooled snapshots and fresh energies and
-        the exchange counts (attempted, accepted)."""
+        the exchange counts (attempted, accepted).  With analysis_atoms (e.g.
+        the Calpha indices), also the pool's mean-distance and
+        contact-frequency matrices of those atoms, computed on the device
+        from the snapshots as they are taken."""
         nseg = max(1, n_snapshots // swap_interval)
         m = nseg * swap_interval
         px = np.zeros((m, self.n, 3))
@@ -293,9 +298,20 @@ class Context:
         sw = np.zeros(2, np.int64)
         p = LptParams(n_replicas, n_snapshots, steps_per_snap, swap_interval, temperature, ladder_ratio, max_angle,
                       seed, 0, exchange_seed)
+        out = {}
+        if analysis_atoms is not None:
+            idx = np.ascontiguousarray(analysis_atoms, dtype=np.int32)
+            k = len(idx)
+            out["mean_dist"], out["contact_freq"] = np.zeros((k, k)), np.zeros((k, k))
+            p.analysis_atoms = _p(idx, C.c_int32)
+            p.n_analysis = k
+            p.contact_cutoff = float(contact_cutoff)
+            p.mean_dist = _p(out["mean_dist"], C.c_double)
+            p.contact_freq = _p(out["contact_freq"], C.c_double)
         self._check(lib().alma_landscape_pt(self._h, C.byref(p), _p(px, C.c_double), _p(pe, C.c_double),
                                             _p(cnt, C.c_int32), sw.ctypes.data_as(C.POINTER(C.c_int64))))
-        return {"snapshots": px[:int(cnt[0])], "energies": pe[:int(cnt[0])], "swaps": sw}
+        out.update({"snapshots": px[:int(cnt[0])], "energies": pe[:int(cnt[0])], "swaps": sw})
+        return out
 
     def ensemble_matrices(self, coords, cutoff: float = 8.0, mean: bool = True, freq: bool = True):
         """compute_mean_dist_matrix / compute_contact_freq_matrix

@@ -87,17 +87,20 @@ void free_analysis(alma_ctx* ctx) {
     dev_free(ctx->an_mean);
     dev_free(ctx->an_freq);
     dev_free(ctx->an_tiles);
+    dev_free(ctx->an_pool);
+    dev_free(ctx->an_pool_idx);
     ctx->an_xyz_cap = ctx->an_mat_cap = 0;
     ctx->an_tiles_n = -1;
     ctx->an_n_tiles = 0;
 }
 
 // Ensemble-mean distance and contact-frequency matrices
-// (src/analysis_ext.cpp:85-167) of (n_snaps, n, 3) coordinates.  dev: the
-// coordinate and matrix pointers are device memory (else host memory,
-// staged through the context's buffers).  mean / freq may be null, not both.
+// (src/analysis_ext.cpp:85-167) of (n_snaps, n, 3) coordinates.  dev_in /
+// dev_out: the coordinate / matrix pointers are device memory (else host
+// memory, staged through the context's buffers).  mean / freq may be null,
+// not both.
 void ensemble(alma_ctx* ctx, const double* coords, int n_snaps, int n, double cutoff, double* mean, double* freq,
-              bool dev) {
+              bool dev_in, bool dev_out) {
     if (n_snaps < 0 || n < 0) throw Fail{ALMA_EINVAL, "coords must be a 3-D array (n_snaps, n_ca, 3)"};
     if (!mean && !freq) throw Fail{ALMA_EINVAL, "no output matrix"};
     if (n_snaps > 0 && n > 0 && !coords) throw Fail{ALMA_EINVAL, "coords missing"};
@@ -108,12 +111,12 @@ void ensemble(alma_ctx* ctx, const double* coords, int n_snaps, int n, double cu
         // no snapshot: zero matrices, as the reference returns them
         for (double* m : {mean, freq}) {
             if (!m || mat == 0) continue;
-            if (dev)
+            if (dev_out)
                 HIP_TRY(hipMemsetAsync(m, 0, sizeof(double) * mat, ctx->stream));
             else
                 std::fill(m, m + mat, 0.0);
         }
-        if (dev) HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (dev_out) HIP_TRY(hipStreamSynchronize(ctx->stream));
         return;
     }
     if (ctx->an_tiles_n != n) {
@@ -127,18 +130,20 @@ void ensemble(alma_ctx* ctx, const double* coords, int n_snaps, int n, double cu
     double* d_mean = mean;
     double* d_freq = freq;
     const size_t nx = (size_t)n_snaps * n * 3;
-    if (!dev) {
+    if (!dev_in) {
         if (nx > ctx->an_xyz_cap) {
             dev_alloc(&ctx->an_xyz, nx);
             ctx->an_xyz_cap = nx;
         }
+        HIP_TRY(hipMemcpyAsync(ctx->an_xyz, coords, sizeof(double) * nx, hipMemcpyHostToDevice, ctx->stream));
+        d_xyz = ctx->an_xyz;
+    }
+    if (!dev_out) {
         if (mat > ctx->an_mat_cap) {
             dev_alloc(&ctx->an_mean, mat);
             dev_alloc(&ctx->an_freq, mat);
             ctx->an_mat_cap = mat;
         }
-        HIP_TRY(hipMemcpyAsync(ctx->an_xyz, coords, sizeof(double) * nx, hipMemcpyHostToDevice, ctx->stream));
-        d_xyz = ctx->an_xyz;
         d_mean = mean ? ctx->an_mean : nullptr;
         d_freq = freq ? ctx->an_freq : nullptr;
     }
@@ -150,7 +155,7 @@ void ensemble(alma_ctx* ctx, const double* coords, int n_snaps, int n, double cu
     HIP_TRY(launch_ensemble(d_xyz, n_snaps, n, cutoff * cutoff, ctx->an_tiles, ctx->an_n_tiles, d_mean, d_freq,
                             ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-    if (!dev) {
+    if (!dev_out) {
         if (mean)
             HIP_TRY(hipMemcpyAsync(mean, d_mean, sizeof(double) * mat, hipMemcpyDeviceToHost, ctx->stream));
         if (freq)
@@ -656,6 +661,10 @@ struct LandscapePt {
     double* pool_xyz = nullptr;   // slot 0: n_seg * snapshots per segment, 3n each
     double* pool_e = nullptr;
     int64_t* swaps = nullptr;     // attempted, accepted
+    // pooled snapshots' analysis atoms, gathered in device memory
+    const int* ana_idx = nullptr;   // device, n_ana
+    int n_ana = 0;
+    double* ana_xyz = nullptr;      // device, pool size * n_ana * 3
 };
 
 void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_stats* stats, double* snap_xyz,
@@ -795,6 +804,9 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
                     for (int i = 0; i < n; ++i)
                         for (int d = 0; d < 3; ++d)
                             snap_xyz[((size_t)c * nsnap + k) * 3 * n + 3 * i + d] = snap_pos[4 * i + d];
+                if (pt && pt->n_ana > 0 && c == 0)
+                    HIP_TRY(launch_gather_xyz(b.pos, pt->ana_idx, pt->n_ana,
+                                              pt->ana_xyz + ((size_t)seg * nsnap + k) * pt->n_ana * 3, ctx->stream));
                 if (snap_e) {
                     HIP_TRY(hipMemcpy(s.pos, snap_pos.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice));
                     double terms[ALMA_NTERMS];
@@ -1841,7 +1853,7 @@ alma_status alma_mc_diagnostic(alma_ctx* ctx, int32_t n_steps, double temperatur
 }
 
 alma_status alma_landscape_pt(alma_ctx* ctx, const alma_landscape_pt_params* p, double* pool_xyz, double* pool_e,
-                              int32_t* n_pool, int64_t* swaps) {
+                              int32_t* n_pool_out, int64_t* swaps) {
     return guarded(ctx, [&] {
         if (!p) throw Fail{ALMA_EINVAL, "parameters missing"};
         if (p->n_replicas <= 0) throw Fail{ALMA_EINVAL, "n_replicas must be positive"};
@@ -1857,6 +1869,21 @@ alma_status alma_landscape_pt(alma_ctx* ctx, const alma_landscape_pt_params* p, 
         pt.pool_xyz = pool_xyz;
         pt.pool_e = pool_e;
         pt.swaps = swaps;
+        if (p->n_analysis < 0) throw Fail{ALMA_EINVAL, "n_analysis must be >= 0"};
+        const int m = p->n_analysis;
+        const int n_pool = pt.n_seg * p->swap_interval;
+        if (m > 0) {
+            if (!p->analysis_atoms) throw Fail{ALMA_EINVAL, "analysis_atoms missing"};
+            if (!p->mean_dist && !p->contact_freq) throw Fail{ALMA_EINVAL, "no output matrix"};
+            for (int a = 0; a < m; ++a)
+                if (p->analysis_atoms[a] < 0 || p->analysis_atoms[a] >= ctx->sys.n)
+                    throw Fail{ALMA_EINVAL, "analysis atom index out of range"};
+            dev_upload(&ctx->an_pool_idx, p->analysis_atoms, (size_t)m);
+            dev_alloc(&ctx->an_pool, (size_t)n_pool * m * 3);
+            pt.ana_idx = ctx->an_pool_idx;
+            pt.n_ana = m;
+            pt.ana_xyz = ctx->an_pool;
+        }
         alma_mc_params mp{};
         mp.mode = ALMA_MC_LANDSCAPE;
         mp.n_chains = p->n_replicas;
@@ -1870,18 +1897,20 @@ alma_status alma_landscape_pt(alma_ctx* ctx, const alma_landscape_pt_params* p, 
         std::vector<double> xyz(3 * (size_t)ctx->sys.n * p->n_replicas);
         std::vector<alma_mc_stats> st(p->n_replicas);
         run_mc(ctx, mp, xyz.data(), st.data(), nullptr, nullptr, &pt);
-        if (n_pool) *n_pool = pt.n_seg * p->swap_interval;
+        if (n_pool_out) *n_pool_out = n_pool;
+        if (m > 0)
+            ensemble(ctx, ctx->an_pool, n_pool, m, p->contact_cutoff, p->mean_dist, p->contact_freq, true, false);
     });
 }
 
 alma_status alma_ensemble_matrices(alma_ctx* ctx, const double* coords, int32_t n_snaps, int32_t n_ca,
                                    double cutoff, double* mean_dist, double* contact_freq) {
-    return guarded(ctx, [&] { ensemble(ctx, coords, n_snaps, n_ca, cutoff, mean_dist, contact_freq, false); });
+    return guarded(ctx, [&] { ensemble(ctx, coords, n_snaps, n_ca, cutoff, mean_dist, contact_freq, false, false); });
 }
 
 alma_status alma_ensemble_matrices_device(alma_ctx* ctx, const double* coords, int32_t n_snaps, int32_t n_ca,
                                           double cutoff, double* mean_dist, double* contact_freq) {
-    return guarded(ctx, [&] { ensemble(ctx, coords, n_snaps, n_ca, cutoff, mean_dist, contact_freq, true); });
+    return guarded(ctx, [&] { ensemble(ctx, coords, n_snaps, n_ca, cutoff, mean_dist, contact_freq, true, true); });
 }
 
 alma_status alma_ensemble_timing(alma_ctx* ctx, double* out) {

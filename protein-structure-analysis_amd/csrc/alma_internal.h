@@ -295,6 +295,8 @@ hipError_t launch_chain_energy(ChainBatch& b, const DevSys& s, hipStream_t st);
 void ensemble_tiles(int n, std::vector<int2>& tiles);
 hipError_t launch_ensemble(const double* xyz, int n_snaps, int n, double cut2, const int2* tiles, int n_tiles,
                            double* mean, double* freq, hipStream_t st);
+// atoms idx[0..m) of an (n, 4) coordinate array into (m, 3)
+hipError_t launch_gather_xyz(const double* pos4, const int* idx, int m, double* out, hipStream_t st);
 
 // ---- kernels / launchers (energy.hip) ----
 hipError_t launch_build_nl(DevSys& s, hipStream_t st);
@@ -358,6 +360,8 @@ struct alma_ctx {
     double *an_xyz = nullptr, *an_mean = nullptr, *an_freq = nullptr;
     size_t an_xyz_cap = 0, an_mat_cap = 0;
     int2* an_tiles = nullptr;
+    double* an_pool = nullptr;      // alma_landscape_pt: pooled snapshots' analysis atoms
+    int* an_pool_idx = nullptr;
     int an_tiles_n = -1, an_n_tiles = 0;
     double t_an_ms = 0.0;
     // MC trace of the last run

@@ -86,7 +86,22 @@ __global__ void __launch_bounds__(kAnBlock)
     }
 }
 
+// atoms idx[0..m) of a (n, 4) coordinate array -> (m, 3)
+__global__ void k_gather_xyz(const double* __restrict__ pos4, const int* __restrict__ idx, int m,
+                             double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * m) return;
+    const int a = k / 3, d = k - 3 * a;
+    out[k] = pos4[4 * (size_t)idx[a] + d];
+}
+
 }  // namespace
+
+hipError_t launch_gather_xyz(const double* pos4, const int* idx, int m, double* out, hipStream_t st) {
+    if (m <= 0) return hipSuccess;
+    k_gather_xyz<<<(3 * m + 255) / 256, 256, 0, st>>>(pos4, idx, m, out);
+    return hipGetLastError();
+}
 
 // Tiles (row block, column block) holding a pair i < j or a diagonal
 // element: every matrix element is written by exactly one tile.

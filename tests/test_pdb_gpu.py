@@ -153,3 +153,18 @@ def test_dropin_run_parallel_tempering():
     x = np.array([[p.x, p.y, p.z] for p in snaps[-1]])
     assert np.max(np.abs(x - o["snapshots"][-1])) < 1e-8
     assert np.all(np.abs(np.array(es) - o["energies"]) <= 1e-5 * np.abs(o["energies"]))
+
+
+def test_landscape_pt_pool_analysis_on_device(ctx):
+    """The pool's Calpha mean-distance / contact-frequency matrices computed
+    on the device while the snapshots are taken equal protein_analysis on the
+    pooled snapshots (oracle loop, bit for bit)."""
+    s, t, c = _parsed("SM_P01308")
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    ca = s["ca_indices"]
+    g = ctx.landscape_pt(3, 4, 10, temperature=KT, ladder_ratio=1.6, swap_interval=2, max_angle=0.12, seed=3,
+                         exchange_seed=1, analysis_atoms=ca, contact_cutoff=7.5)
+    om, of = O.ensemble_matrices(g["snapshots"][:, ca, :], 7.5)
+    assert np.array_equal(g["mean_dist"], om) and np.array_equal(g["contact_freq"], of)
+    with pytest.raises(ValueError):
+        ctx.landscape_pt(2, 2, 5, temperature=KT, swap_interval=1, analysis_atoms=[0, len(s["xyz"])])
