@@ -1066,9 +1066,12 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     const double* pjo = pos + 4 * j;
                     const double* pin = gi ? posn + 4 * i : pio;
                     const double* pjn = gj ? posn + 4 * j : pjo;
+                    // both radii are loaded with the coordinates (one round
+                    // trip), then ordered by index
+                    const float ri_ = s.prm[i].y, rj_ = s.prm[j].y;
                     float r2o = (float)dist2_exact(pio, pjo), r2n = (float)dist2_exact(pin, pjn);
-                    int lo = min(i, j), hi = max(i, j);
-                    float rl = s.prm[lo].y, rh = s.prm[hi].y;
+                    int lo = min(i, j);
+                    float rl = lo == i ? ri_ : rj_, rh = lo == i ? rj_ : ri_;
                     double dlt = (double)hct_f(sqrtf(r2n), r2n, rl, rh) - (double)hct_f(sqrtf(r2o), r2o, rl, rh);
                     // SASA contact range (with margin) in either geometry
                     float dc = rl + rh + (float)(2.0 * kProbe) + 0.01f;
