@@ -290,8 +290,9 @@ class Context:
         the Calpha indices), also the pool's mean-distance and
         contact-frequency matrices of those atoms, computed on the device
         from the snapshots as they are taken."""
-        nseg = max(1, n_snapshots // swap_interval)
-        m = nseg * swap_interval
+        # (arguments are validated by the library: size the buffers for any)
+        nseg = max(1, n_snapshots // swap_interval) if swap_interval > 0 else 1
+        m = max(nseg * swap_interval, 1)
         px = np.zeros((m, self.n, 3))
         pe = np.zeros(m)
         cnt = np.zeros(1, np.int32)

@@ -168,3 +168,47 @@ def test_landscape_pt_pool_analysis_on_device(ctx):
     assert np.array_equal(g["mean_dist"], om) and np.array_equal(g["contact_freq"], of)
     with pytest.raises(ValueError):
         ctx.landscape_pt(2, 2, 5, temperature=KT, swap_interval=1, analysis_atoms=[0, len(s["xyz"])])
+
+
+@pytest.mark.parametrize("r,ns,sps,si", [(1, 4, 10, 2), (2, 3, 12, 5), (5, 6, 8, 2)])
+def test_landscape_pt_shapes_match_oracle(ctx, r, ns, sps, si):
+    """One slot (no exchanges), a swap interval longer than the run (one
+    segment of swap_interval snapshots, as max(1, n // k) gives), five slots
+    over three segments (both pair offsets)."""
+    s, t, c = _parsed("SM_P01308")
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    print("landscape pt", r, ns, sps, si, flush=True)
+    g = ctx.landscape_pt(r, ns, sps, temperature=KT, ladder_ratio=1.4, swap_interval=si, max_angle=0.12, seed=7,
+                         exchange_seed=11)
+    o = O.landscape_pt(s, c, r, ns, sps, KT, 1.4, si, 0.12, seed=7, exchange_seed=11, null_floor=N.NULL_DE)
+    assert len(g["energies"]) == max(1, ns // si) * si
+    assert np.array_equal(g["swaps"], o["swaps"])
+    assert np.max(np.abs(g["snapshots"] - o["snapshots"])) < 1e-8
+    assert np.all(np.abs(g["energies"] - o["energies"]) <= 1e-5 * np.abs(o["energies"]))
+
+
+def test_landscape_pt_capacity_growth(ctx, monkeypatch):
+    """Row capacities far too small at the start: the chains grow them and
+    rebuild, and the run equals the oracle's as with ample capacities."""
+    s, t, c = _parsed("SM_P01308")
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    monkeypatch.setenv("ALMA_MC_TEST_CAPS", "64,0")
+    g = ctx.landscape_pt(3, 4, 10, temperature=KT, ladder_ratio=1.6, swap_interval=2, max_angle=0.12, seed=5,
+                         exchange_seed=2)
+    assert ctx.mc_timing()["regrows"] >= 1
+    o = O.landscape_pt(s, c, 3, 4, 10, KT, 1.6, 2, 0.12, seed=5, exchange_seed=2, null_floor=N.NULL_DE)
+    assert np.array_equal(g["swaps"], o["swaps"])
+    assert np.max(np.abs(g["snapshots"] - o["snapshots"])) < 1e-8
+
+
+def test_landscape_pt_bad_arguments(ctx):
+    s, t, c = _parsed("SM_P01308")
+    ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
+    for kw in ({"n_replicas": 0}, {"n_snapshots": 0}, {"steps_per_snap": 0}, {"swap_interval": 0}):
+        args = dict(n_replicas=2, n_snapshots=2, steps_per_snap=5, swap_interval=1)
+        args.update(kw)
+        with pytest.raises(ValueError):
+            ctx.landscape_pt(args["n_replicas"], args["n_snapshots"], args["steps_per_snap"], temperature=KT,
+                             swap_interval=args["swap_interval"])
+    with pytest.raises(ValueError):
+        ctx.landscape_pt(2, 2, 5, temperature=-1.0, swap_interval=1)
