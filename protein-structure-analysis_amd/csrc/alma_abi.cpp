@@ -10,6 +10,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "alma_internal.h"
@@ -465,7 +466,40 @@ void alloc_rows(ChainBatch& b) {
     dev_alloc(&b.bm, (size_t)b.nch * 2 * 16 * (size_t)((b.n + 31) / 32));   // per-wavefront row bitmaps + counts
 }
 
+// Per-chain buffers of a run of nch chains.  The buffers of the previous
+// run are kept when they fit (same atom and chain counts, cell tables at
+// least as large): every run initialises what it reads (coordinates, words,
+// chain states are uploaded, the first list build of each chain sets the
+// lists, radii and folds), and a batch of several GB is not returned to and
+// taken from the runtime on every call.
 void alloc_chains(ChainBatch& b, const DevSys& s, int nch, int trace_steps) {
+    const bool reuse = b.pos && b.n == s.n && b.nch == nch && b.max_cells >= s.max_cells;
+    if (reuse) {
+        const size_t N = (size_t)nch * s.n;
+        HIP_TRY(hipMemset(b.dhalf, 0, sizeof(double) * N));
+        HIP_TRY(hipMemset(b.flag, 0, sizeof(int) * N));
+        if (trace_steps != b.trace_cap) {
+            dev_free(b.tr_kind);
+            dev_free(b.tr_bond);
+            dev_free(b.tr_delta);
+            dev_free(b.tr_dE);
+            dev_free(b.tr_acc);
+            dev_free(b.tr_parts);
+            dev_free(b.tr_hc);
+            b.trace_cap = 0;
+            if (trace_steps > 0) {
+                b.trace_cap = trace_steps;
+                dev_alloc(&b.tr_kind, trace_steps);
+                dev_alloc(&b.tr_bond, trace_steps);
+                dev_alloc(&b.tr_delta, trace_steps);
+                dev_alloc(&b.tr_dE, trace_steps);
+                dev_alloc(&b.tr_acc, trace_steps);
+                dev_alloc(&b.tr_parts, 4 * (size_t)trace_steps);
+                dev_alloc(&b.tr_hc, trace_steps);
+            }
+        }
+        return;
+    }
     free_chains(b);
     b.n = s.n;
     b.nch = nch;
@@ -686,31 +720,46 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     int h_flags[2];
     HIP_TRY(hipMemcpy(h_flags, s.flags, sizeof(h_flags), hipMemcpyDeviceToHost));
     ChainBatch& b = ctx->chains;
+    const char* test_caps = std::getenv("ALMA_MC_TEST_CAPS");
+    if (test_caps) free_chains(b);   // the growth test starts from fresh, small buffers
     alloc_chains(b, s, nch, trace_steps);
     int maxh = std::max(h_flags[1], 32);
-    b.cap_h = (maxh * 3 / 2 + 63) / 64 * 64;
-    b.cap_f = pow2_at_least(3 * maxh);
-    b.cap_s = (std::max(64, maxh) + 63) / 64 * 64;
-    if (const char* tc = std::getenv("ALMA_MC_TEST_CAPS")) {
+    const int need_h = (maxh * 3 / 2 + 63) / 64 * 64;
+    const int need_f = pow2_at_least(3 * maxh);
+    const int need_s = (std::max(64, maxh) + 63) / 64 * 64;
+    if (test_caps) {
         // testing: start from the given row capacity and cell-grid size, so
         // that the growth path runs ("rows,cells")
+        b.cap_h = need_h;
+        b.cap_f = need_f;
+        b.cap_s = need_s;
         int rows = 0, cells = 0;
-        if (std::sscanf(tc, "%d,%d", &rows, &cells) >= 1 && rows > 0) b.cap_h = b.cap_f = b.cap_s = round64(rows);
+        if (std::sscanf(test_caps, "%d,%d", &rows, &cells) >= 1 && rows > 0) b.cap_h = b.cap_f = b.cap_s = round64(rows);
         if (cells > 0) {
             b.max_cells = cells;
             dev_alloc(&b.cell_cnt, (size_t)nch * (b.max_cells + 1));
             dev_alloc(&b.cell_start, (size_t)nch * (b.max_cells + 1));
         }
+        alloc_rows(b);
+    } else if (!b.nlh || b.cap_h < need_h || b.cap_f < need_f || b.cap_s < need_s) {
+        // rows of a previous run are kept when wide enough (a wider row only
+        // leaves room unused)
+        b.cap_h = std::max(b.cap_h, need_h);
+        b.cap_f = std::max(b.cap_f, need_f);
+        b.cap_s = std::max(b.cap_s, need_s);
+        alloc_rows(b);
     }
-    alloc_rows(b);
     // initial state of every chain
     {
         std::vector<double> p4(4 * (size_t)n);
         for (int i = 0; i < n; ++i)
             for (int d = 0; d < 3; ++d) p4[4 * i + d] = ctx->h_pos[3 * i + d], p4[4 * i + 3] = 0.0;
-        for (int c = 0; c < nch; ++c)
-            HIP_TRY(hipMemcpy(b.pos + (size_t)c * 4 * n, p4.data(), sizeof(double) * p4.size(),
-                              hipMemcpyHostToDevice));
+        // one upload, then device-to-device copies for the other chains
+        HIP_TRY(hipMemcpy(b.pos, p4.data(), sizeof(double) * p4.size(), hipMemcpyHostToDevice));
+        for (int c = 1; c < nch; ++c)
+            HIP_TRY(hipMemcpyAsync(b.pos + (size_t)c * 4 * n, b.pos, sizeof(double) * p4.size(),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     std::vector<ChainState> st(nch);
     std::vector<HostMT> mt;
@@ -724,11 +773,24 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     std::vector<std::vector<uint32_t>> wbuf(nch, std::vector<uint32_t>(b.wcap));
     std::vector<long long> consumed(nch, 0);
     mt.reserve(nch);
+    for (int c = 0; c < nch; ++c) mt.emplace_back(p.seed + (uint32_t)c);
+    {
+        // each chain's first words, generated by host threads (the chains'
+        // generators are independent), uploaded in one copy
+        std::vector<uint32_t> all((size_t)nch * b.wcap);
+        const int nt = std::max(1, std::min(nch, std::min(16, (int)std::thread::hardware_concurrency())));
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (int c = t; c < nch; c += nt) {
+                    for (auto& w : wbuf[c]) w = mt[c]();
+                    std::copy(wbuf[c].begin(), wbuf[c].end(), all.begin() + (size_t)c * b.wcap);
+                }
+            });
+        for (auto& x : th) x.join();
+        HIP_TRY(hipMemcpy(b.words, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice));
+    }
     for (int c = 0; c < nch; ++c) {
-        mt.emplace_back(p.seed + (uint32_t)c);
-        for (auto& w : wbuf[c]) w = mt[c]();
-        HIP_TRY(hipMemcpy(b.words + (size_t)c * b.wcap, wbuf[c].data(), sizeof(uint32_t) * b.wcap,
-                          hipMemcpyHostToDevice));
         ChainState& x = st[c];
         x = ChainState{};
         x.cur_max = p.max_angle;
@@ -927,6 +989,20 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
         std::fprintf(stderr, "[alma mc] pass B us/step: cross pairs %.2f SASA %.2f dihedral+SS %.2f accept %.2f\n",
                      x.clk[8] / 100.0 / x.steps_done, x.clk[9] / 100.0 / x.steps_done,
                      x.clk[10] / 100.0 / x.steps_done, x.clk[5] / 100.0 / x.steps_done);
+        {
+            // spread of the chains' busy time (steps + list builds): the
+            // launch lasts as long as its slowest chain
+            double lo = 1e300, hi = 0.0, sum = 0.0;
+            for (int c = 0; c < nch; ++c) {
+                long long t = 0;
+                for (int k = 0; k < 11; ++k) t += st[c].clk[k];
+                lo = std::min(lo, (double)t);
+                hi = std::max(hi, (double)t);
+                sum += (double)t;
+            }
+            std::fprintf(stderr, "[alma mc] chain busy ms: min %.1f mean %.1f max %.1f\n", lo / 1e5, sum / nch / 1e5,
+                         hi / 1e5);
+        }
         long long nb = x.rebuilds + 1;
         std::fprintf(stderr, "[alma mc] list build us: grid %.1f cells %.1f fill %.1f born %.1f sasa %.1f | per-wave sums: "
                      "atom setup %.1f candidate scan %.1f full-row conversion %.1f\n",
