@@ -101,3 +101,18 @@ def test_bad_arguments(ctx):
         ctx.ensemble_matrices(np.zeros((3, 4, 3)), mean=False, freq=False)
     with pytest.raises(ValueError):
         ctx.ensemble_matrices_device(0, 2, 4, 8.0, 0, 0)
+
+
+def test_special_distances(ctx):
+    """Coincident atoms (distance 0), subnormal and near-overflow squared
+    distances: the square root's fallback path, bit-equal to the oracle."""
+    x = np.zeros((3, 6, 3))
+    x[:, 1] = x[:, 0]                       # coincident
+    x[:, 2] = [1e-160, 0.0, 0.0]            # d^2 ~ 1e-320 (subnormal)
+    x[:, 3] = [3e-155, 2e-155, 0.0]         # d^2 below 2^-767
+    x[:, 4] = [1e150, -1e150, 5e149]        # d^2 ~ 2e300
+    x[:, 5] = [0.7, 1.3, -2.2]
+    x[1] *= 1.0000001
+    m, f = ctx.ensemble_matrices(x)
+    om, of = O.ensemble_matrices(x)
+    assert np.array_equal(m, om) and np.array_equal(f, of)
