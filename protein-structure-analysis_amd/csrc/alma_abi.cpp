@@ -804,6 +804,12 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     }
     const int nsnap = p.snapshot_every > 0 ? p.steps / p.snapshot_every : 0;
     std::vector<int> snaps_done(nch, 0);
+    // few chains: their list builds run over many workgroups (kr_*), the MC
+    // kernel stops a chain whose build is due (CS_NEEDBUILD); a full batch
+    // builds inside the MC kernel, every CU being busy with a chain
+    const bool split = nch * 4 <= ctx->n_cu && split_build_fits(n) && !std::getenv("ALMA_MC_INKERNEL_BUILD");
+    b.ext_build = split ? 1 : 0;
+    const int split_g = std::max(1, std::min((2 * ctx->n_cu) / std::max(nch, 1), (n + 3) / 4));
     auto target_of = [&](int c) {
         if (nsnap && snaps_done[c] < nsnap) return std::min(p.steps, (snaps_done[c] + 1) * p.snapshot_every);
         return p.steps;
@@ -828,6 +834,27 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     for (int seg = 0; seg < (pt ? pt->n_seg : 1); ++seg) {
     for (int round = 0;; ++round) {
         HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
+        if (split) {
+            // due list builds first (a run's first build uncounted, as inside
+            // the kernel), then the round starts over from the new states
+            std::vector<int> bl;
+            for (int c = 0; c < nch; ++c) {
+                if (st[c].status == CS_REBUILD) bl.push_back(c);
+                if (st[c].status == CS_NEEDBUILD) bl.push_back((int)((unsigned)c | kBuildCountBit));
+            }
+            if (!bl.empty()) {
+                HIP_TRY(hipMemcpyAsync(b.list, bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice, ctx->stream));
+                HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+                HIP_TRY(launch_split_build(b, s, (int)bl.size(), split_g, ctx->stream));
+                HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+                HIP_TRY(hipEventSynchronize(ctx->ev1));
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                ctx->t_rebuild_ms += ms;
+                ctx->n_rebuild_launch += 1;
+                continue;
+            }
+        }
         std::vector<int> run;
         bool dirty = false;
         bool grow = false;

@@ -101,7 +101,7 @@ struct DevSys {
 // ---- device MC (mc.hip) ----
 // Chain status codes written by the MC kernel.
 enum { CS_RUN = 0, CS_REBUILD = 1, CS_TARGET = 2, CS_WORDS = 3, CS_ERROR = 4, CS_GRID = 5, CS_CAPACITY = 6,
-       CS_STUCK = 7 };
+       CS_STUCK = 7, CS_NEEDBUILD = 8 /* a list build is due; the host runs it over many workgroups */ };
 
 // State of one MC chain (device resident).
 struct ChainState {
@@ -136,6 +136,7 @@ struct ChainState {
 // Per-chain arrays are chain-major with the strides below.
 struct ChainBatch {
     int n = 0, nch = 0;
+    int ext_build = 0;        // list builds of running chains are left to the host (CS_NEEDBUILD)
     int cap_h = 0, cap_f = 0, cap_s = 0, wcap = 0, max_cells = 0;
     double* pos = nullptr;    // nch * 4n current coordinates
     double* posn = nullptr;   // nch * 4n trial coordinates of moved atoms
@@ -290,6 +291,11 @@ hipError_t launch_batch_pair(const Batch& b, int lanes, int block, hipStream_t s
 hipError_t launch_chain_mc(ChainBatch& b, const DevSys& s, int nlist, bool diag, hipStream_t st);
 // curE of every chain: total energy at its last list build + accepted dE since
 hipError_t launch_chain_energy(ChainBatch& b, const DevSys& s, hipStream_t st);
+// list builds of the L chains in b.list (bit 30: counted as a rebuild) over
+// G workgroups per chain; false when the system is too large for it
+constexpr unsigned kBuildCountBit = 0x40000000u;
+bool split_build_fits(int n);
+hipError_t launch_split_build(ChainBatch& b, const DevSys& s, int L, int G, hipStream_t st);
 // ensemble metrics (analysis.hip): tiles of the (n x n) matrices, and the
 // kernel over (n_snaps, n, 3) coordinates; mean / freq may be null
 void ensemble_tiles(int n, std::vector<int2>& tiles);

@@ -119,22 +119,24 @@ constexpr int kLdsCells = 1024;   // cell-start tables up to this size are read 
 // The FP32 distance (relative coordinates below ~100 A: |error of d^2| <
 // 1e-3 A^2) decides every pair except those within 0.01 A^2 of the 196 A^2
 // radius, which take the reference's FP64 test; the SASA-row reach carries
-// the same margin.  kLds: stage, cell starts and row bitmap in LDS, so the
-// candidate loop issues no global loads and never waits on its stores.
-template <bool kLds>
+// the same margin.  kLds: row bitmap (and its prefix counts) in LDS;
+// kLdsStage: stage and cell starts in LDS too, so the candidate loop issues
+// no global loads and never waits on its stores.  The wavefront builds the
+// rows of atoms i_first, i_first + i_step, ...
+template <bool kLds, bool kLdsStage = kLds>
 __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, int c, const Grid& gr,
                                           const int* cell_of, const int* start, const float4* cp, const int* ci,
                                           uint32_t* rb, int* pref, int& max_h, int& max_s, int& max_f,
-                                          int& ovf, long long& sum_h, long long& sum_s) {
+                                          int& ovf, long long& sum_h, long long& sum_s, int i_first, int i_step) {
     const int n = b.n;
-    const int lane = lane_id(), wave = threadIdx.x / kWave;
+    const int lane = lane_id();
     const int W = (n + 31) / 32;
     const double* pos = b.pos + (size_t)c * 4 * n;
     const DevTopo& T = s.topo;
     const bool topo = s.has_topo;
     const float fcs = (float)gr.cs;
     long long t_scan = 0, t_conv = 0, t_pre = 0;
-    for (int i = wave; i < n; i += kMcWaves) {
+    for (int i = i_first; i < n; i += i_step) {
         long long ta = wall_clock64();
         const double* pid = pos + 4 * i;
         const float pix = (float)(pid[0] - gr.lo[0]), piy = (float)(pid[1] - gr.lo[1]),
@@ -179,7 +181,7 @@ __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, 
                     }
                     const int cc = x + gr.nc[0] * (y + gr.nc[1] * z);
                     int b0, e;
-                    if (kLds) {
+                    if (kLdsStage) {
                         b0 = ((lds_ci32*)start)[cc];
                         e = ((lds_ci32*)start)[cc + 1];
                     } else {
@@ -193,7 +195,7 @@ __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, 
                         int j = i;
                         if (k < e) {
                             float4 q;
-                            if (kLds) {
+                            if (kLdsStage) {
                                 j = ((lds_ci32*)ci)[k];
                                 lds_cf32* qp = (lds_cf32*)cp + 4 * k;
                                 q = make_float4(qp[0], qp[1], qp[2], qp[3]);
@@ -332,35 +334,29 @@ __device__ __forceinline__ void fill_rows(const ChainBatch& b, const DevSys& s, 
 // stage: n float4 (cell-ordered coordinates relative to the grid origin in
 // FP32, w = R*) followed by n ints (atom of each cell slot); LDS when the
 // chain's atoms fit (kLdsRebuild), global scratch otherwise
-__device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int count_rebuild, double* dhalf,
-                              int* flag, float4* cp, int* ci, uint32_t* rbits, bool rbits_lds) {
+// First phase of a chain's list build (NeighborList::build,
+// src/physics_engine.cpp:293-342), one workgroup of kMcBlock threads: Born
+// sums and flags cleared, bounding box -> cell grid, cell of every atom,
+// cell starts, atoms in ascending index inside each cell (the reference's
+// push_back order) and the cell-ordered FP32 stage (cp: coordinates relative
+// to the grid origin, w = R*; ci: atom of each slot).  sstart receives an
+// LDS copy of the starts when the grid has fewer than kLdsCells cells
+// (lds_cells).
+template <class Tick>
+__device__ __forceinline__ Grid build_cells(const ChainBatch& b, const DevSys& s, int c, float4* cp, int* ci,
+                                             int* sstart, bool& lds_cells, Tick&& rtick) {
     const int n = b.n;
     const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-    const int W = (n + 31) / 32;
     const double* pos = b.pos + (size_t)c * 4 * n;
-    double* ref = b.ref + (size_t)c * 4 * n;
     Grid* g = b.grid + c;
     int* fl = b.flags + 4 * c;
     int* cnt = b.cell_cnt + (size_t)c * (b.max_cells + 1);
     int* start = b.cell_start + (size_t)c * (b.max_cells + 1);
     int* cell_of = b.cell_of + R1(b, c);
     int* tmp = (int*)cp;   // slot scratch before the coordinates are staged
-    uint32_t* rb = rbits + (size_t)wave * W;                          // this wavefront's row bitmap
-    int* pref = (int*)(rbits + (size_t)(kMcWaves + wave) * W);        // ... and its word prefix counts
     double* sum = b.born_sum + R1(b, c);
-    const DevTopo& T = s.topo;
     __shared__ double slo[3][kMcWaves], shi[3][kMcWaves];
     __shared__ int sbuf[kMcBlock];
-    __shared__ int sstart[kLdsCells + 1];
-    long long t0 = tid == 0 ? wall_clock64() : 0;
-    auto rtick = [&](int ph) {
-        if (tid == 0) {
-            long long now = wall_clock64();
-            b.st[c].rclk[ph] += now - t0;
-            t0 = now;
-        }
-    };
-
     // clear
     for (int i = tid; i < n; i += kMcBlock) sum[i] = 0.0;
     for (int k = tid; k <= b.max_cells; k += kMcBlock) cnt[k] = 0;
@@ -407,6 +403,7 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
     rtick(0);
     const Grid gr = *g;
     const bool grid_ok = gr.ncell <= b.max_cells;
+    (void)grid_ok;
     // cell of every atom
     for (int i = tid; i < n; i += kMcBlock) {
         int id = 0;
@@ -452,7 +449,7 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
             for (int k = tid; k <= ncell; k += kMcBlock) sstart[k] = start[k];
         __syncthreads();
     }
-    const bool lds_cells = min(gr.ncell, b.max_cells) < kLdsCells;
+    lds_cells = min(gr.ncell, b.max_cells) < kLdsCells;
     for (int i = tid; i < n; i += kMcBlock) {
         int cl = cell_of[i];
         tmp[start[cl] + atomicAdd(&cnt[cl], 1)] = i;
@@ -475,15 +472,48 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
     }
     __syncthreads();
     rtick(1);
+    return gr;
+}
+
+__device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int count_rebuild, double* dhalf,
+                              int* flag, float4* cp, int* ci, uint32_t* rbits, bool rbits_lds) {
+    const int n = b.n;
+    const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+    const int W = (n + 31) / 32;
+    const double* pos = b.pos + (size_t)c * 4 * n;
+    double* ref = b.ref + (size_t)c * 4 * n;
+    Grid* g = b.grid + c;
+    int* fl = b.flags + 4 * c;
+    int* cnt = b.cell_cnt + (size_t)c * (b.max_cells + 1);
+    int* start = b.cell_start + (size_t)c * (b.max_cells + 1);
+    int* cell_of = b.cell_of + R1(b, c);
+    uint32_t* rb = rbits + (size_t)wave * W;                          // this wavefront's row bitmap
+    int* pref = (int*)(rbits + (size_t)(kMcWaves + wave) * W);        // ... and its word prefix counts
+    double* sum = b.born_sum + R1(b, c);
+    const DevTopo& T = s.topo;
+    __shared__ int sstart[kLdsCells + 1];
+    long long t0 = tid == 0 ? wall_clock64() : 0;
+    auto rtick = [&](int ph) {
+        if (tid == 0) {
+            long long now = wall_clock64();
+            b.st[c].rclk[ph] += now - t0;
+            t0 = now;
+        }
+    };
+
+    bool lds_cells = false;
+    const Grid gr = build_cells(b, s, c, cp, ci, sstart, lds_cells, rtick);
+    const bool grid_ok = gr.ncell <= b.max_cells;
     const bool topo = s.has_topo;
     int max_h = 0, max_s = 0, max_f = 0, ovf = 0;
     long long sum_h = 0, sum_s = 0;
     if (grid_ok) {
         if (rbits_lds && lds_cells)
-            fill_rows<true>(b, s, c, gr, cell_of, sstart, cp, ci, rb, pref, max_h, max_s, max_f, ovf, sum_h, sum_s);
+            fill_rows<true>(b, s, c, gr, cell_of, sstart, cp, ci, rb, pref, max_h, max_s, max_f, ovf, sum_h, sum_s,
+                            wave, kMcWaves);
         else
             fill_rows<false>(b, s, c, gr, cell_of, start, cp, ci, rb, pref, max_h, max_s, max_f, ovf, sum_h,
-                             sum_s);
+                             sum_s, wave, kMcWaves);
     }
     (void)topo;
     if (lane == 0) {
@@ -565,6 +595,117 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
         else cs->status = CS_RUN;
     }
     wg_sync_global();
+}
+
+// ── list builds over many workgroups ──────────────────────────────────────
+// A run of few chains builds their lists here rather than inside the MC
+// kernel (one workgroup per chain): kr_cells (one workgroup per chain: grid,
+// cells, stage), kr_rows and kr_born (G workgroups per chain, a wavefront
+// per atom: the rows, as fill_rows builds them, and the HCT sums of the
+// half rows), kr_finish (radii, build positions, SASA folds), kr_state
+// (chain status).  Same lists and folds as chain_rebuild, radii up to the
+// order of the atomic HCT sums (as there).  b.list holds the chains,
+// kBuildCountBit marking builds counted as rebuilds.
+constexpr int kRbBlock = 256;
+constexpr int kRbWaves = kRbBlock / kWave;
+constexpr size_t kRbMaxLds = 150 * 1024;
+
+__global__ void __launch_bounds__(kMcBlock) kr_cells(ChainBatch b, DevSys s) {
+    const int c = (int)((unsigned)b.list[blockIdx.x] & ~kBuildCountBit);
+    __shared__ int sstart[kLdsCells + 1];
+    bool lds_cells = false;
+    (void)build_cells(b, s, c, b.stage + R1(b, c), b.stage_idx + R1(b, c), sstart, lds_cells, [](int) {});
+}
+
+__global__ void __launch_bounds__(kRbBlock) kr_rows(ChainBatch b, DevSys s, int G) {
+    const int k = blockIdx.x / G, g = blockIdx.x - k * G;
+    const int c = (int)((unsigned)b.list[k] & ~kBuildCountBit);
+    const Grid gr = b.grid[c];
+    if (gr.ncell > b.max_cells) return;
+    const int W = (b.n + 31) / 32, wave = threadIdx.x / kWave;
+    extern __shared__ uint32_t lbits[];
+    uint32_t* rb = lbits + (size_t)wave * W;
+    int* pref = (int*)(lbits + (size_t)(kRbWaves + wave) * W);
+    int max_h = 0, max_s = 0, max_f = 0, ovf = 0;
+    long long sum_h = 0, sum_s = 0;
+    fill_rows<true, false>(b, s, c, gr, b.cell_of + R1(b, c), b.cell_start + (size_t)c * (b.max_cells + 1),
+                           b.stage + R1(b, c), b.stage_idx + R1(b, c), rb, pref, max_h, max_s, max_f, ovf, sum_h,
+                           sum_s, g * kRbWaves + wave, G * kRbWaves);
+    if (lane_id() == 0) {
+        int* fl = b.flags + 4 * c;
+        atomicMax(&fl[1], max_h);
+        atomicMax(&fl[2], max_f);
+        atomicMax(&fl[3], max_s);
+        if (ovf) atomicOr(&fl[0], ovf);
+        atomicAdd((unsigned long long*)&b.st[c].n_build_pairs, (unsigned long long)sum_h);
+        atomicAdd((unsigned long long*)&b.st[c].n_build_sasa, (unsigned long long)sum_s);
+    }
+}
+
+// HCT sums (born_radii :2024-2039: every half-row pair, both directions)
+__global__ void __launch_bounds__(kRbBlock) kr_born(ChainBatch b, DevSys s, int G) {
+    const int k = blockIdx.x / G, g = blockIdx.x - k * G;
+    const int c = (int)((unsigned)b.list[k] & ~kBuildCountBit);
+    if (b.grid[c].ncell > b.max_cells) return;
+    const int n = b.n, lane = lane_id(), wave = threadIdx.x / kWave;
+    const double* pos = b.pos + (size_t)c * 4 * n;
+    double* sum = b.born_sum + R1(b, c);
+    for (int i = g * kRbWaves + wave; i < n; i += G * kRbWaves) {
+        const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+        const float ri = s.prm[i].y;
+        const int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
+        const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
+        double acc = 0.0;
+        for (int q = lane; q < m; q += kWave) {
+            const int j = (int)(row[q] & kIdxMask);
+            const float r2 = (float)dist2_exact(pi, pos + 4 * j), r = sqrtf(r2);
+            const float rj = s.prm[j].y;
+            acc += (double)hct_f(r, r2, ri, rj);
+            atomicAdd(&sum[j], (double)hct_f(r, r2, rj, ri));
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) {
+            atomicAdd(&sum[i], acc);
+            b.half[R1(b, c) + i] = acc;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kRbBlock) kr_finish(ChainBatch b, DevSys s, int G) {
+    const int k = blockIdx.x / G, g = blockIdx.x - k * G;
+    const int c = (int)((unsigned)b.list[k] & ~kBuildCountBit);
+    if (b.grid[c].ncell > b.max_cells) return;
+    const int n = b.n, lane = lane_id(), wave = threadIdx.x / kWave;
+    const double* pos = b.pos + (size_t)c * 4 * n;
+    double* ref = b.ref + (size_t)c * 4 * n;
+    const double* sum = b.born_sum + R1(b, c);
+    for (int i = g * kRbBlock + (int)threadIdx.x; i < n; i += G * kRbBlock) {
+        const size_t o = R1(b, c) + i;
+        b.a[o] = 1.0 / fmax(1.0 / s.rstar[i] - 0.5 * sum[i], 2.0);
+        for (int d = 0; d < 4; ++d) ref[4 * i + d] = pos[4 * i + d];
+        b.dhalf[o] = 0.0;
+        b.flag[o] = 0;
+    }
+    for (int i = g * kRbWaves + wave; i < n; i += G * kRbWaves) {
+        const double pi[3] = {pos[4 * i], pos[4 * i + 1], pos[4 * i + 2]};
+        const double ri = (double)s.prm[i].y + kProbe;
+        const int ms = min(b.nls_cnt[R1(b, c) + i], b.cap_s);
+        int clamped;
+        const double sa = sasa_fold_wave(pi, pos, s.prm, ri, b.nls + (R1(b, c) + i) * (size_t)b.cap_s, ms, &clamped);
+        if (lane == 0) b.sa_cur[R1(b, c) + i] = sa;
+    }
+}
+
+__global__ void kr_state(ChainBatch b) {
+    if (threadIdx.x != 0) return;
+    const unsigned e = (unsigned)b.list[blockIdx.x];
+    const int c = (int)(e & ~kBuildCountBit);
+    ChainState* cs = b.st + c;
+    cs->de_since = 0.0;
+    cs->rb_pending = 0;
+    if (e & kBuildCountBit) cs->rebuilds += 1;
+    const int f = b.flags[4 * c];
+    cs->status = (f & 2) ? CS_GRID : (f & 5) ? CS_CAPACITY : CS_RUN;
 }
 
 // ───────────────────────────────── MC step ──────────────────────────────────
@@ -789,6 +930,15 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                 pending = 0;
             }
             __syncthreads();
+        }
+        if (pending == 2 && b.ext_build) {
+            // the host builds the lists over many workgroups (few chains)
+            if (tid == 0) {
+                cs->status = CS_NEEDBUILD;
+                cs->rb_pending = 0;
+            }
+            __syncthreads();
+            break;
         }
         if (pending) {
             if (tid == 0) t_prev = wall_clock64();
@@ -1540,6 +1690,27 @@ hipError_t launch_chain_energy(ChainBatch& b, const DevSys& s, hipStream_t st) {
     ce_radii<<<g1, kBlock, 0, st>>>(b, s);
     ce_terms<<<gw, kBlock, 0, st>>>(b, s);
     ce_total<<<b.nch, kBlock, 0, st>>>(b, s);
+    return hipGetLastError();
+}
+
+bool split_build_fits(int n) { return (size_t)2 * kRbWaves * ((n + 31) / 32) * sizeof(uint32_t) <= kRbMaxLds; }
+
+hipError_t launch_split_build(ChainBatch& b, const DevSys& s, int L, int G, hipStream_t st) {
+    if (L <= 0) return hipSuccess;
+    const size_t lds = (size_t)2 * kRbWaves * ((b.n + 31) / 32) * sizeof(uint32_t);
+    if (lds > kRbMaxLds) return hipErrorInvalidValue;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)kr_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kRbMaxLds);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    kr_cells<<<L, kMcBlock, 0, st>>>(b, s);
+    kr_rows<<<L * G, kRbBlock, lds, st>>>(b, s, G);
+    kr_born<<<L * G, kRbBlock, 0, st>>>(b, s, G);
+    kr_finish<<<L * G, kRbBlock, 0, st>>>(b, s, G);
+    kr_state<<<L, 64, 0, st>>>(b);
     return hipGetLastError();
 }
 
