@@ -63,6 +63,30 @@ def test_trajectory_matches_oracle(ctx, mode, steps):
     _compare(gpu, orc, steps, "mode%d" % mode)
 
 
+@pytest.mark.parametrize("mode,steps", [(N.MC_ENSEMBLE, 300), (N.MC_LANDSCAPE, 300)])
+def test_trajectory_in_kernel_build(ctx, monkeypatch, mode, steps):
+    """The same trajectories with every list build inside the MC kernel
+    (the path of full batches; runs of few chains build over many
+    workgroups): same moves, decisions, rebuild count, coordinates."""
+    monkeypatch.setenv("ALMA_MC_INKERNEL_BUILD", "1")
+    test_trajectory_matches_oracle(ctx, mode, steps)
+
+
+def test_full_batch_builds_in_kernel(ctx):
+    """A batch of more chains than a quarter of the CUs (in-kernel list
+    builds): first and last chain against the oracle."""
+    s = S.chain(30, 5)
+    c = S.csr(S.topology(s))
+    _load(ctx, s, c)
+    nch = 300
+    gpu = ctx.mc_run(N.MC_ENSEMBLE, 120, n_chains=nch, temperature=KT, seed=501)
+    for k in (0, nch - 1):
+        orc = O.mc_run(s, c, 120, KT, 0.12, seed=501 + k, null_floor=N.NULL_DE)
+        assert gpu["stats"][k]["words"] == orc["stats"]["words"]
+        assert gpu["stats"][k]["rebuilds"] == orc["stats"]["rebuilds"]
+        assert np.max(np.abs(gpu["xyz"][k] - orc["xyz"])) < 1e-8
+
+
 def test_chains_use_consecutive_seeds(ctx):
     s = S.chain(30, 5)
     t = S.topology(s)
