@@ -809,7 +809,8 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     // builds inside the MC kernel, every CU being busy with a chain
     const bool split = nch * 4 <= ctx->n_cu && split_build_fits(n) && !std::getenv("ALMA_MC_INKERNEL_BUILD");
     b.ext_build = split ? 1 : 0;
-    const int split_g = std::max(1, std::min((2 * ctx->n_cu) / std::max(nch, 1), (n + 3) / 4));
+    // workgroups per chain of a build round: about two per CU in all
+    auto split_g = [&](int chains) { return std::max(1, std::min((2 * ctx->n_cu) / std::max(chains, 1), (n + 3) / 4)); };
     auto target_of = [&](int c) {
         if (nsnap && snaps_done[c] < nsnap) return std::min(p.steps, (snaps_done[c] + 1) * p.snapshot_every);
         return p.steps;
@@ -845,7 +846,7 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
             if (!bl.empty()) {
                 HIP_TRY(hipMemcpyAsync(b.list, bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice, ctx->stream));
                 HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-                HIP_TRY(launch_split_build(b, s, (int)bl.size(), split_g, ctx->stream));
+                HIP_TRY(launch_split_build(b, s, (int)bl.size(), split_g((int)bl.size()), ctx->stream));
                 HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
                 HIP_TRY(hipEventSynchronize(ctx->ev1));
                 float ms = 0.f;
