@@ -201,8 +201,11 @@ alma_status alma_landscape_pt(alma_ctx* ctx, const alma_landscape_pt_params* p, 
 
 /* Measurement of the last alma_mc_run (no reference counterpart): out[12] =
  * {device MC kernel ms (HIP events on the context stream), MC launches,
- *  chain-time spent in list builds (ms, summed over chains; builds run
- *  inside the MC kernel), list rebuilds, cross pairs, in-cutoff pair
+ *  time spent in list builds (ms: chain-time of builds inside the MC
+ *  kernel summed over chains, plus the device time of the build rounds of
+ *  runs with few chains, which build over many workgroups), list rebuilds
+ *  (builds after drift; a run's first builds are not counted), cross pairs,
+ *  in-cutoff pair
  *  evaluations, SASA row entries folded, moved atoms, boundary dihedral
  *  records, list-build half-row pairs, list-build SASA-row entries},
  *  summed over chains, then the number of times neighbour-row capacities or

@@ -852,7 +852,6 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
                 float ms = 0.f;
                 HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
                 ctx->t_rebuild_ms += ms;
-                ctx->n_rebuild_launch += 1;
                 continue;
             }
         }
@@ -986,8 +985,9 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
     for (int k = 0; k < 7; ++k) ctx->work[k] = 0;
-    // list rebuilds run inside the MC kernel: chain-time spent in them
-    // (wall clock of each chain's workgroup) and their count
+    // list rebuilds: chain-time of those inside the MC kernel (wall clock of
+    // each chain's workgroup; the device time of builds over many workgroups
+    // was added per build round) and their count
     for (int c = 0; c < nch; ++c) {
         ctx->t_rebuild_ms += st[c].clk[7] / 1e5;
         ctx->n_rebuild_launch += st[c].rebuilds;
