@@ -6,8 +6,8 @@ the tests), torsion Metropolis MC at 300 K with single-torsion +-30 deg
 uniform moves, every dE evaluated by the device-resident MC kernel.  One
 bench step = one batch: `--chains` independent chains (one workgroup each,
 chain c seeded mt19937(seed + c)) advancing `--mc-steps` MC steps each from
-the loaded structure; 512 chains = two per CU, so that CUs whose first chain
-finished early (fewer list builds) take a second one.  value = dE evaluations per second over the timed
+the loaded structure; 1024 chains = four per CU, so that CUs whose chains
+finished early (fewer list builds) take the next ones.  value = dE evaluations per second over the timed
 steps (chains x MC steps / wall time), inputs resident in HBM.
 
 Also reported: the single-chain rate (one chain, latency-bound), the
@@ -51,9 +51,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--chains", type=int, default=512,
-                    help="chains per batch: 512 = two per CU, the second half dispatched as CUs free up "
-                         "(chain times vary with their number of list builds)")
+    ap.add_argument("--chains", type=int, default=1024,
+                    help="chains per batch: 1024 = four per CU, dispatched as CUs free up (chain times vary "
+                         "with their number of list builds)")
     ap.add_argument("--mc-steps", type=int, default=200)
     ap.add_argument("--single-steps", type=int, default=1000)
     ap.add_argument("--residues", type=int, default=300)
