@@ -540,8 +540,11 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
         const int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
         const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
         double acc = 0.0;
+        uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
         for (int k = lane; k < m; k += kWave) {
-            const int j = (int)(row[k] & kIdxMask);
+            const uint32_t e = e_next;
+            if (k + kWave < m) e_next = row[k + kWave];
+            const int j = (int)(e & kIdxMask);
             const float r2 = (float)dist2_exact(pi, pos + 4 * j), r = sqrtf(r2);
             const float rj = s.prm[j].y;
             acc += (double)hct_f(r, r2, ri, rj);
@@ -656,8 +659,11 @@ __global__ void __launch_bounds__(kRbBlock) kr_born(ChainBatch b, DevSys s, int 
         const int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
         const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
         double acc = 0.0;
+        uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
         for (int q = lane; q < m; q += kWave) {
-            const int j = (int)(row[q] & kIdxMask);
+            const uint32_t e = e_next;
+            if (q + kWave < m) e_next = row[q + kWave];
+            const int j = (int)(e & kIdxMask);
             const float r2 = (float)dist2_exact(pi, pos + 4 * j), r = sqrtf(r2);
             const float rj = s.prm[j].y;
             acc += (double)hct_f(r, r2, ri, rj);
@@ -1379,12 +1385,17 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                         row = b.nls + (R1(b, c) + i) * (size_t)b.cap_s;
                     }
                     cnt_sasa += lane == 0 ? m : 0;
+                    // the next chunk's row entries are loaded while this
+                    // chunk's partners are gathered and folded
+                    uint32_t e_next = lane < m ? row[lane] : 0u;
                     for (int base = 0; base < m; base += kWave) {
                         const int k = base + lane;
+                        const uint32_t e_cur = e_next;
+                        if (k + kWave < m) e_next = row[k + kWave];
                         double cap_n = 0.0;
                         bool cn = false;
                         if (k < m) {
-                            const int j = (int)(row[k] & kIdxMask);
+                            const int j = (int)(e_cur & kIdxMask);
                             const double dc = ri + (double)s.prm[j].y + kProbe;
                             const double* pjn = group(j) ? posn + 4 * j : pos + 4 * j;
                             const double r2 = dist2_exact(pin, pjn);

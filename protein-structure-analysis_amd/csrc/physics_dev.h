@@ -271,9 +271,13 @@ __device__ inline double sasa_fold_wave(const double* pi, const double* pos, con
     const double k2 = 2.0 * M_PI * ri * ri / (2.0 * ri);
     double sa = 4.0 * M_PI * ri * ri;
     int clamped = 0;
+    // the next chunk's row entries are loaded while this chunk is folded
+    uint32_t e_next = lane < m ? row[lane] : 0u;
     for (int base = 0; base < m; base += kWave) {
+        const uint32_t e = e_next;
+        if (base + kWave + lane < m) e_next = row[base + kWave + lane];
         SasaContact c{false, 0, 0.0};
-        if (base + lane < m) c = sasa_contact(pi, pos, prm, ri, k2, row[base + lane]);
+        if (base + lane < m) c = sasa_contact(pi, pos, prm, ri, k2, e);
         uint64_t mask = __ballot(c.hit);
         while (mask) {
             int b = __ffsll((unsigned long long)mask) - 1;
@@ -289,72 +293,6 @@ __device__ inline double sasa_fold_wave(const double* pi, const double* pos, con
     }
     *n_clamped = clamped;
     return sa;
-}
-
-// Two atoms' folds by one wavefront: the row entries and partner gathers of
-// both are issued before either is used (indices clamped into the rows, so
-// every load is in bounds and the code is branch-free up to the folds), and
-// the two serial folds are stepped together.  Same arithmetic as
-// sasa_fold_wave for each atom.
-__device__ inline void sasa_fold_wave2(const double* pia, const double* pib, const double* pos, const float4* prm,
-                                       double ria, double rib, int ia, int ib, const uint32_t* rowa, int ma,
-                                       const uint32_t* rowb, int mb, double* sa_a, double* sa_b) {
-    const int lane = lane_id();
-    const double k2a = 2.0 * M_PI * ria * ria / (2.0 * ria), k2b = 2.0 * M_PI * rib * rib / (2.0 * rib);
-    double sa = 4.0 * M_PI * ria * ria, sb = 4.0 * M_PI * rib * rib;
-    const int mm = ma > mb ? ma : mb;
-    for (int base = 0; base < mm; base += kWave) {
-        const int k = base + lane;
-        const bool va = k < ma, vb = k < mb;
-        const int ja = ma > 0 ? (int)(rowa[va ? k : ma - 1] & kIdxMask) : ia;
-        const int jb = mb > 0 ? (int)(rowb[vb ? k : mb - 1] & kIdxMask) : ib;
-        const double* qa = pos + 4 * ja;
-        const double* qb = pos + 4 * jb;
-        const double xa = qa[0], ya = qa[1], za = qa[2], xb = qb[0], yb = qb[1], zb = qb[2];
-        const float rja = prm[ja].y, rjb = prm[jb].y;
-        double capa = 0.0, capb = 0.0;
-        bool ha = false, hb = false;
-        {
-#pragma clang fp contract(off)
-            const double dxa = pia[0] - xa, dya = pia[1] - ya, dza = pia[2] - za;
-            const double dxb = pib[0] - xb, dyb = pib[1] - yb, dzb = pib[2] - zb;
-            const double r2a = dxa * dxa + dya * dya + dza * dza, r2b = dxb * dxb + dyb * dyb + dzb * dzb;
-            const double dca = ria + (double)rja + kProbe, dcb = rib + (double)rjb + kProbe;
-            if (va && r2a < dca * dca) {
-                const double r = (double)sqrtf((float)r2a);
-                ha = r < dca;
-                capa = k2a * (dca - r);
-            }
-            if (vb && r2b < dcb * dcb) {
-                const double r = (double)sqrtf((float)r2b);
-                hb = r < dcb;
-                capb = k2b * (dcb - r);
-            }
-        }
-        uint64_t ma_ = __ballot(ha), mb_ = __ballot(hb);
-        while (ma_ | mb_) {
-            if (ma_) {
-                const int bit = __ffsll((unsigned long long)ma_) - 1;
-                const double cp = lane_value(capa, bit);
-                if (sa * 0.85 < cp)
-                    sa -= sa * 0.85;
-                else
-                    sa -= cp;
-                ma_ &= ma_ - 1ull;
-            }
-            if (mb_) {
-                const int bit = __ffsll((unsigned long long)mb_) - 1;
-                const double cp = lane_value(capb, bit);
-                if (sb * 0.85 < cp)
-                    sb -= sb * 0.85;
-                else
-                    sb -= cp;
-                mb_ &= mb_ - 1ull;
-            }
-        }
-    }
-    *sa_a = sa;
-    *sa_b = sb;
 }
 
 // Adjoint of the fold for a wavefront: each unclamped contact carries
