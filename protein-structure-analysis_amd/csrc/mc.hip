@@ -1156,7 +1156,12 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     for (int q = wave; q < ns; q += kMcWaves) {
                         int4 g = segs[q];
                         const uint32_t* row = nlf + (size_t)g.x * b.cap_f;
-                        for (int p = g.y + lane; p < g.z; p += kWave) fn(g.x, g.w, row[p], 1 - g.w, true);
+                        uint32_t e_next = g.y + lane < g.z ? row[g.y + lane] : 0u;   // next entries in flight
+                        for (int p = g.y + lane; p < g.z; p += kWave) {
+                            const uint32_t e = e_next;
+                            if (p + kWave < g.z) e_next = row[p + kWave];
+                            fn(g.x, g.w, e, 1 - g.w, true);
+                        }
                         fn(g.x, g.w, 0u, 0, false);
                     }
                     return;
