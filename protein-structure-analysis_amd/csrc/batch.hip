@@ -466,8 +466,11 @@ __global__ void __launch_bounds__(kBlock) kb_born(Batch b) {
     int m = b.nlf_cnt[g];
     const uint32_t* row = b.nlf + (size_t)g * b.cap_f;
     double acc = 0.0;
+    uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
     for (int k = lane; k < m; k += kWave) {
-        int j = (int)(row[k] & kIdxMask);
+        const uint32_t e_cur = e_next;
+        if (k + kWave < m) e_next = row[k + kWave];
+        int j = (int)(e_cur & kIdxMask);
         double r2d = dist2_exact(pi, x + 4 * j);
         float r2 = (float)r2d, r = sqrtf(r2);
         acc += (double)hct_f(r, r2, ri, prm[j].y);
@@ -632,8 +635,11 @@ __global__ void __launch_bounds__(kBlock) kb_bornchain(Batch b) {
     int m = b.nlf_cnt[g];
     const uint32_t* row = b.nlf + (size_t)g * b.cap_f;
     double g0 = 0, g1 = 0, g2 = 0;
+    uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
     for (int k = lane; k < m; k += kWave) {
-        int j = (int)(row[k] & kIdxMask);
+        const uint32_t e_cur = e_next;
+        if (k + kWave < m) e_next = row[k + kWave];
+        int j = (int)(e_cur & kIdxMask);
         double aj = A[j];
         double wj = aj < 0.5 ? dEda[j] * 0.5 * aj * aj : 0.0;
         if (wi == 0.0 && wj == 0.0) continue;

@@ -204,8 +204,11 @@ __global__ void __launch_bounds__(kBlock) k_born_sum(const double* __restrict__ 
     int m = min(cnt[i], cap);
     const uint32_t* row = nl + (size_t)i * cap;
     double acc = 0.0;
+    uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
     for (int k = lane; k < m; k += kWave) {
-        int j = (int)(row[k] & kIdxMask);
+        const uint32_t e_cur = e_next;
+        if (k + kWave < m) e_next = row[k + kWave];
+        int j = (int)(e_cur & kIdxMask);
         double r2d = dist2_exact(pi, pos + 4 * j);
         float r2 = (float)r2d, r = sqrtf(r2);
         float rj = prm[j].y;
@@ -244,8 +247,10 @@ __global__ void __launch_bounds__(kBlock) k_pair_energy(const double* __restrict
     int m = min(cnt[i], cap);
     const uint32_t* row = nl + (size_t)i * cap;
     double dh = 0, gb = 0, lj = 0, hc = 0;
+    uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
     for (int k = lane; k < m; k += kWave) {
-        uint32_t e = row[k];
+        const uint32_t e = e_next;
+        if (k + kWave < m) e_next = row[k + kWave];
         if (use_excl && (e & kExclBit)) continue;
         int j = (int)(e & kIdxMask);
         double r2d = dist2_exact(pi, pos + 4 * j);

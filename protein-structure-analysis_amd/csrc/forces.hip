@@ -84,8 +84,10 @@ __global__ void __launch_bounds__(kBlock) k_f_pair(DevSys s, double* __restrict_
     int m = min(s.nl_cnt[i], s.cap);
     const uint32_t* row = s.nl + (size_t)i * s.cap;
     double gi[3] = {0, 0, 0}, dai = 0.0, dh = 0, gb = 0, lj = 0, hc = 0;
+    uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
     for (int k = lane; k < m; k += kWave) {
-        uint32_t e = row[k];
+        const uint32_t e = e_next;
+        if (k + kWave < m) e_next = row[k + kWave];
         if (s.has_topo && (e & kExclBit)) continue;
         int j = (int)(e & kIdxMask);
         double r2 = dist2_exact(pi, pos + 4 * j);
@@ -140,8 +142,11 @@ __global__ void __launch_bounds__(kBlock) k_f_born(DevSys s, const double* __res
     int m = min(s.nl_cnt[i], s.cap);
     const uint32_t* row = s.nl + (size_t)i * s.cap;
     double gi[3] = {0, 0, 0};
+    uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
     for (int k = lane; k < m; k += kWave) {
-        int j = (int)(row[k] & kIdxMask);
+        const uint32_t e_cur = e_next;
+        if (k + kWave < m) e_next = row[k + kWave];
+        int j = (int)(e_cur & kIdxMask);
         double wj = weight(j);
         if (wi == 0.0 && wj == 0.0) continue;
         double r2 = dist2_exact(pi, pos + 4 * j), r = sqrt(r2);

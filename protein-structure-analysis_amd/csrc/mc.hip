@@ -1617,8 +1617,11 @@ __global__ void __launch_bounds__(kBlock) ce_born(ChainBatch b, DevSys s) {
     const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
     double* sum = b.born_sum + R1(b, c);
     double acc = 0.0;
+    uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
     for (int k = lane; k < m; k += kWave) {
-        int j = (int)(row[k] & kIdxMask);
+        const uint32_t e_cur = e_next;
+        if (k + kWave < m) e_next = row[k + kWave];
+        int j = (int)(e_cur & kIdxMask);
         float r2 = (float)dist2_exact(pi, ref + 4 * j), r = sqrtf(r2);
         float rj = s.prm[j].y;
         acc += (double)hct_f(r, r2, ri, rj);
@@ -1647,8 +1650,10 @@ __global__ void __launch_bounds__(kBlock) ce_terms(ChainBatch b, DevSys s) {
     int m = min(b.nlh_cnt[R1(b, c) + i], b.cap_h);
     const uint32_t* row = b.nlh + (R1(b, c) + i) * (size_t)b.cap_h;
     double acc = 0.0;
+    uint32_t e_next = lane < m ? row[lane] : 0u;   // next row entries in flight
     for (int k = lane; k < m; k += kWave) {
-        uint32_t e = row[k];
+        const uint32_t e = e_next;
+        if (k + kWave < m) e_next = row[k + kWave];
         if (e & kExclBit) continue;
         int j = (int)(e & kIdxMask);
         double r2 = dist2_exact(pi, ref + 4 * j);
