@@ -1172,8 +1172,11 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     const uint32_t* row = nlf + (size_t)i * b.cap_f;
                     int cnt = min(nlf_cnt[i], b.cap_f);
                     if (mode == 2) {
+                        uint32_t e_next = lane < cnt ? row[lane] : 0u;   // next row entries in flight
                         for (int p = lane; p < cnt; p += kWave) {
-                            uint32_t e = row[p];
+                            const uint32_t e_cur = e_next;
+                            if (p + kWave < cnt) e_next = row[p + kWave];
+                            uint32_t e = e_cur;
                             int j = (int)(e & kIdxMask);
                             int gj = group(j);
                             if (gj == gi || (gj != 0 && j < i)) continue;
@@ -1193,8 +1196,11 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                             p0 = lower_bound_row(row, 0, cnt, r1.r[q].x);
                             p1 = lower_bound_row(row, p0, cnt, r1.r[q].y + 1);
                         }
+                        uint32_t e_next = p0 + lane < p1 ? row[p0 + lane] : 0u;   // next row entries in flight
                         for (int p = p0 + lane; p < p1; p += kWave) {
-                            uint32_t e = row[p];
+                            const uint32_t e_cur = e_next;
+                            if (p + kWave < p1) e_next = row[p + kWave];
+                            uint32_t e = e_cur;
                             fn(i, gi, e, 1 - gi, true);
                         }
                     }
@@ -1267,8 +1273,11 @@ __global__ void __launch_bounds__(kMcBlock) k_mc(ChainBatch b, DevSys s) {
                     for (int q = 0; q < rp.n; ++q) {
                         int p0 = lower_bound_row(row, 0, cnt, rp.r[q].x);
                         int p1 = lower_bound_row(row, p0, cnt, rp.r[q].y + 1);
+                        uint32_t e_next = p0 + lane < p1 ? row[p0 + lane] : 0u;   // next row entries in flight
                         for (int p = p0 + lane; p < p1; p += kWave) {
-                            int j = (int)(row[p] & kIdxMask);
+                            const uint32_t e_cur = e_next;
+                            if (p + kWave < p1) e_next = row[p + kWave];
+                            int j = (int)(e_cur & kIdxMask);
                             float r2o = (float)dist2_exact(pos + 4 * i, pos + 4 * j);
                             float r2n = (float)dist2_exact(posn + 4 * i, posn + 4 * j);
                             int lo = min(i, j), hi = max(i, j);
