@@ -510,8 +510,10 @@ __global__ void kb_pair(Batch b) {
     int m = live ? b.nlf_cnt[g] : 0;
     const uint32_t* row = b.nlf + (size_t)g * b.cap_f;
     double dh = 0, gb = 0, lj = 0, hc = 0, gi0 = 0, gi1 = 0, gi2 = 0, dai = 0;
+    uint32_t e_next = sub < m ? row[sub] : 0u;   // next row entries in flight
     for (int k = sub; k < m; k += L) {
-        uint32_t e = row[k];
+        const uint32_t e = e_next;
+        if (k + L < m) e_next = row[k + L];
         if (e & kExclBit) continue;
         int j = (int)(e & kIdxMask);
         double r2 = dist2_exact(pi, x + 4 * j);
