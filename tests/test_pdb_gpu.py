@@ -74,7 +74,8 @@ def test_forces_match_oracle(ctx):
     assert np.all(np.abs(gt - GT) <= 1e-5 * (np.abs(GT) + np.sqrt(np.mean(GT ** 2))))
 
 
-@pytest.mark.parametrize("name,mode", [("1XQ8", N.MC_ENSEMBLE), ("SM_P01308", N.MC_LANDSCAPE)])
+@pytest.mark.parametrize("name,mode", [("1XQ8", N.MC_ENSEMBLE), ("SM_P01308", N.MC_LANDSCAPE),
+                                       ("SM_P21543", N.MC_ENSEMBLE)])
 def test_mc_trajectory_matches_oracle(ctx, name, mode):
     s, t, c = _parsed(name)
     ctx.set_system(s["xyz"], s["charge"], s["rstar"], s["eps"], c)
