@@ -1038,12 +1038,33 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
                      x.rclk[5] / 100.0 / nb, x.rclk[6] / 100.0 / nb / 16, x.rclk[7] / 100.0 / nb / 16,
                      x.rclk[3] / 100.0 / nb / 16);
     }
-    // results
-    std::vector<double> p4(4 * (size_t)n);
+    // results: every chain's coordinates in one copy into pinned staging
+    // (kept by the context), unpacked by host threads
+    {
+        const size_t tot = (size_t)nch * 4 * n;
+        if (ctx->h_pin_cap < tot) {
+            if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
+            ctx->h_pin = nullptr;
+            ctx->h_pin_cap = 0;
+            HIP_TRY(hipHostMalloc((void**)&ctx->h_pin, sizeof(double) * tot, hipHostMallocDefault));
+            ctx->h_pin_cap = tot;
+        }
+        HIP_TRY(hipMemcpy(ctx->h_pin, b.pos, sizeof(double) * tot, hipMemcpyDeviceToHost));
+        const double* src = ctx->h_pin;
+        const int nt = std::max(1, std::min(nch, std::min(16, (int)std::thread::hardware_concurrency())));
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (int c = t; c < nch; c += nt) {
+                    const double* q = src + (size_t)c * 4 * n;
+                    double* o = xyz_out + (size_t)c * 3 * n;
+                    for (int i = 0; i < n; ++i)
+                        for (int d = 0; d < 3; ++d) o[3 * i + d] = q[4 * i + d];
+                }
+            });
+        for (auto& x : th) x.join();
+    }
     for (int c = 0; c < nch; ++c) {
-        HIP_TRY(hipMemcpy(p4.data(), b.pos + (size_t)c * 4 * n, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i)
-            for (int d = 0; d < 3; ++d) xyz_out[(size_t)c * 3 * n + 3 * i + d] = p4[4 * i + d];
         stats[c].cur_e = st[c].cur_e;
         stats[c].cur_max = st[c].cur_max;
         stats[c].accepted = st[c].accepted;
@@ -1766,6 +1787,7 @@ void alma_destroy(alma_ctx* ctx) {
     free_forces(ctx);
     free_batch(ctx);
     free_analysis(ctx);
+    if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);

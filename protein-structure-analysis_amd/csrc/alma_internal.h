@@ -370,6 +370,9 @@ struct alma_ctx {
     int* an_pool_idx = nullptr;
     int an_tiles_n = -1, an_n_tiles = 0;
     double t_an_ms = 0.0;
+    // pinned staging of MC results (all chains' coordinates), grown on demand
+    double* h_pin = nullptr;
+    size_t h_pin_cap = 0;
     // MC trace of the last run
     std::vector<int> tr_kind, tr_bond, tr_acc;
     std::vector<double> tr_delta, tr_dE, tr_parts, tr_hc;
