@@ -776,19 +776,30 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     for (int c = 0; c < nch; ++c) mt.emplace_back(p.seed + (uint32_t)c);
     {
         // each chain's first words, generated by host threads (the chains'
-        // generators are independent), uploaded in one copy
-        std::vector<uint32_t> all((size_t)nch * b.wcap);
+        // generators are independent) into the context's pinned staging,
+        // uploaded in one copy
+        const size_t nw = (size_t)nch * b.wcap;
+        const size_t need = std::max((nw * sizeof(uint32_t) + sizeof(double) - 1) / sizeof(double),
+                                     (size_t)nch * 4 * n);
+        if (ctx->h_pin_cap < need) {
+            if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
+            ctx->h_pin = nullptr;
+            ctx->h_pin_cap = 0;
+            HIP_TRY(hipHostMalloc((void**)&ctx->h_pin, sizeof(double) * need, hipHostMallocDefault));
+            ctx->h_pin_cap = need;
+        }
+        uint32_t* all = reinterpret_cast<uint32_t*>(ctx->h_pin);
         const int nt = std::max(1, std::min(nch, std::min(16, (int)std::thread::hardware_concurrency())));
         std::vector<std::thread> th;
         for (int t = 0; t < nt; ++t)
             th.emplace_back([&, t] {
                 for (int c = t; c < nch; c += nt) {
                     for (auto& w : wbuf[c]) w = mt[c]();
-                    std::copy(wbuf[c].begin(), wbuf[c].end(), all.begin() + (size_t)c * b.wcap);
+                    std::copy(wbuf[c].begin(), wbuf[c].end(), all + (size_t)c * b.wcap);
                 }
             });
         for (auto& x : th) x.join();
-        HIP_TRY(hipMemcpy(b.words, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b.words, all, sizeof(uint32_t) * nw, hipMemcpyHostToDevice));
     }
     for (int c = 0; c < nch; ++c) {
         ChainState& x = st[c];
