@@ -34,7 +34,7 @@ struct Fail {
 template <class T>
 void dev_alloc(T** p, size_t count) {
     if (*p) {
-        hipFree(*p);
+        (void)hipFree(*p);
         *p = nullptr;
     }
     if (count == 0) count = 1;
@@ -49,7 +49,7 @@ void dev_upload(T** p, const T* src, size_t count) {
 
 template <class T>
 void dev_free(T*& p) {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
 }
 
@@ -566,10 +566,10 @@ void widen_rows(T** p, size_t rows, int old_cap, int new_cap) {
     hipError_t e = hipMemcpy2D(q, sizeof(T) * new_cap, *p, sizeof(T) * old_cap, sizeof(T) * old_cap, rows,
                                hipMemcpyDeviceToDevice);
     if (e != hipSuccess) {
-        hipFree(q);
+        (void)hipFree(q);
         HIP_TRY(e);
     }
-    hipFree(*p);
+    (void)hipFree(*p);
     *p = q;
 }
 
@@ -844,7 +844,7 @@ void run_mc(alma_ctx* ctx, const alma_mc_params& p, double* xyz_out, alma_mc_sta
     std::vector<double> cm(nch, p.max_angle);
     int64_t att = 0, acc = 0;
     for (int seg = 0; seg < (pt ? pt->n_seg : 1); ++seg) {
-    for (int round = 0;; ++round) {
+    for (;;) {
         HIP_TRY(hipMemcpy(st.data(), b.st, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
         if (split) {
             // due list builds first (a run's first build uncounted, as inside
@@ -1502,14 +1502,14 @@ struct PtGraph {
         HIP_TRY(ce);
         hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (ie != hipSuccess) {
-            hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             graph = nullptr;
             HIP_TRY(ie);
         }
     }
     ~PtGraph() {
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
     }
 };
 
@@ -1792,7 +1792,7 @@ alma_status alma_create(int device, alma_ctx** out) {
 
 void alma_destroy(alma_ctx* ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     free_sys(ctx->sys);
     free_chains(ctx->chains);
     free_forces(ctx);
@@ -1801,7 +1801,7 @@ void alma_destroy(alma_ctx* ctx) {
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 

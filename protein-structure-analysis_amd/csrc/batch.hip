@@ -86,7 +86,7 @@ __device__ __forceinline__ void rotate(const double* p, const double* o, const d
 }
 
 // u . sum over the bond's side atoms of (x_i - o) x G_i  (block reduction)
-__device__ double side_torque(const Batch& b, const BSys& sy, const BProt& pr, int k, const double* x,
+__device__ double side_torque(const Batch& b, const BProt& pr, int k, const double* x,
                               const double* G, const double* o, const double* u, double* sh) {
     double acc = 0.0;
     for (int q = b.run_off[pr.rot_off + k]; q < b.run_off[pr.rot_off + k + 1]; ++q) {
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(kSysBlock) kb_propose(Batch b, int init) {
     const int k = sk;
     if (svalid) {
         double o[3] = {so[0], so[1], so[2]}, u[3] = {su[0], su[1], su[2]};
-        double g = side_torque(b, sy, pr, k, x, G, o, u, sh);
+        double g = side_torque(b, pr, k, x, G, o, u, sh);
         if (t == 0) {
             double beta = 1.0 / st.temperature;
             double mu = -0.5 * b.h * b.h * beta * g;
@@ -767,7 +767,7 @@ __global__ void __launch_bounds__(kSysBlock) kb_accept(Batch b, int init) {
         double o[3], u[3];
         int k = st.bond;
         bond_axis(xn, b.rb_i[pr.rot_off + k], b.rb_j[pr.rot_off + k], o, u);
-        gn = side_torque(b, sy, pr, k, xn, Gn, o, u, sh);
+        gn = side_torque(b, pr, k, xn, Gn, o, u, sh);
     }
     if (t == 0) {
         double* tm = b.terms + 8 * (size_t)s;

@@ -29,7 +29,6 @@ using namespace dev;
 
 constexpr int kMcBlock = 1024;
 constexpr int kMcWaves = kMcBlock / kWave;
-constexpr int kSortBlock = 64;
 constexpr double kSasaSkin = 4.0;    // A beyond contact kept in the SASA sub-row
 constexpr double kSasaTrial = 1.5;   // max trial displacement served by it
 constexpr int kLdsRuns = 16;         // moved-set runs cached in LDS
@@ -37,9 +36,6 @@ constexpr int kSegRuns = 3;          // runs handled by the segment path
 constexpr int kLdsAtoms = 11264;     // systems up to this size keep per-step atom state in LDS
 constexpr int kLdsRebuild = 5632;    // ... and stage the list build in LDS (about 24 B per atom)
 
-__device__ __forceinline__ double* P4(const ChainBatch& b, double* base, int c) {
-    return base + (size_t)c * 4 * b.n;
-}
 __device__ __forceinline__ size_t R1(const ChainBatch& b, int c) { return (size_t)c * b.n; }
 
 __device__ __forceinline__ float4 atom4(const DevSys& s, const double* a, int i) {
@@ -482,15 +478,12 @@ __device__ void chain_rebuild(const ChainBatch& b, const DevSys& s, int c, int c
     const int W = (n + 31) / 32;
     const double* pos = b.pos + (size_t)c * 4 * n;
     double* ref = b.ref + (size_t)c * 4 * n;
-    Grid* g = b.grid + c;
     int* fl = b.flags + 4 * c;
-    int* cnt = b.cell_cnt + (size_t)c * (b.max_cells + 1);
     int* start = b.cell_start + (size_t)c * (b.max_cells + 1);
     int* cell_of = b.cell_of + R1(b, c);
     uint32_t* rb = rbits + (size_t)wave * W;                          // this wavefront's row bitmap
     int* pref = (int*)(rbits + (size_t)(kMcWaves + wave) * W);        // ... and its word prefix counts
     double* sum = b.born_sum + R1(b, c);
-    const DevTopo& T = s.topo;
     __shared__ int sstart[kLdsCells + 1];
     long long t0 = tid == 0 ? wall_clock64() : 0;
     auto rtick = [&](int ph) {
