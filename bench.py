@@ -6,7 +6,7 @@ the tests), torsion Metropolis MC at 300 K with single-torsion +-30 deg
 uniform moves, every dE evaluated by the device-resident MC kernel.  One
 bench step = one batch: `--chains` independent chains (one workgroup each,
 chain c seeded mt19937(seed + c)) advancing `--mc-steps` MC steps each from
-the loaded structure; 1024 chains = four per CU, so that CUs whose chains
+the loaded structure; 2048 chains = eight per CU, so that CUs whose chains
 finished early (fewer list builds) take the next ones.  value = dE evaluations per second over the timed
 steps (chains x MC steps / wall time), inputs resident in HBM.
 
@@ -51,8 +51,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--chains", type=int, default=1024,
-                    help="chains per batch: 1024 = four per CU, dispatched as CUs free up (chain times vary "
+    ap.add_argument("--chains", type=int, default=2048,
+                    help="chains per batch: 2048 = eight per CU, dispatched as CUs free up (chain times vary "
                          "with their number of list builds)")
     ap.add_argument("--mc-steps", type=int, default=200)
     ap.add_argument("--single-steps", type=int, default=1000)
@@ -139,6 +139,9 @@ def main():
     # single chain, same mode (latency-bound)
     single = None
     if args.single_steps > 0:
+        # untimed warm-up call: the change of batch shape frees the batch's
+        # chain buffers and allocates the single chain's
+        ctx.mc_run(N.MC_TORSION30, 20, n_chains=1, temperature=kt, seed=76)
         t1 = time.perf_counter()
         ctx.mc_run(N.MC_TORSION30, args.single_steps, n_chains=1, temperature=kt, seed=77)
         single = args.single_steps / (time.perf_counter() - t1)
